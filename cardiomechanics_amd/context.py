@@ -1,0 +1,229 @@
+"""Host-side handle on one GPU mechanics context (a thin layer over include/cmg.h).
+
+`MechanicsContext` plays the role of the reference's CBSolver + CBFormulation pair
+for one mesh: the element loop (CBFormulation::CalcNodalForces /
+CalcNodalForcesJacobian, CBFormulation.h:30-41) and the solver step
+(CBSolver::SolverStep, CBSolver.h:214) run in libcmg.so on the GPU; this class
+only marshals numpy arrays across the C ABI and raises on error codes.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from .materials import Material
+from .mesh import Mesh
+
+STATUS_NAMES = {2: "SUCCESS", 0: "NOTHING_DONE", -2: "REPEAT", -3: "FAILED", -4: "INFINITIVE",
+                -5: "NOT_A_NUMBER", -6: "CORRUPT_ELEMENT", -8: "SYSTEM_WRONG_STATE",
+                -100: "ERR_INVALID", -101: "ERR_HIP", -102: "ERR_NOMEM"}
+
+
+class CmgError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__(f"{where}: {STATUS_NAMES.get(code, code)} ({_lib.last_error()})")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def default_options(**kw) -> _lib.CmgOptions:
+    o = _lib.CmgOptions()
+    _lib.load().cmg_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def to_cmg_material(m: Material) -> _lib.CmgMaterial:
+    r = _lib.CmgMaterial()
+    r.law = int(m.law)
+    r.ignore_corrupt = int(m.ignore_corrupt)
+    r.tension_fn = int(m.tension_fn)
+    r.density = float(m.density)
+    for i in range(12):
+        r.params[i] = float(m.params[i]) if i < len(m.params) else 0.0
+        r.fn[i] = float(m.fn[i]) if i < len(m.fn) else 0.0
+    r.tension_max = float(m.tension_max)
+    return r
+
+
+class MechanicsContext:
+    def __init__(self, mesh: Mesh, materials, **options):
+        self.L = _lib.load()
+        self.mesh = mesh
+        self.materials = list(materials)
+        self.opt = default_options(**options)
+        self.fp32 = self.opt.precision == _lib.PREC_MIXED
+        E, npe = mesh.n_elements, mesh.npe
+        # keep the arrays alive while the context is created
+        self._coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+        self._elements = np.ascontiguousarray(mesh.elements, dtype=np.int32)
+        nb = 5 if npe == 10 else 1
+        self._bases = np.ascontiguousarray(mesh.bases.reshape(E, nb, 9), dtype=np.float64)
+        self._matidx = np.ascontiguousarray(mesh.material, dtype=np.int32)
+        self._tact = np.ascontiguousarray(mesh.activation, dtype=np.float64)
+        self._bc = np.ascontiguousarray(mesh.bc, dtype=np.uint8)
+        self._surf = np.ascontiguousarray(mesh.surfaces, dtype=np.int32)
+        groups = np.ascontiguousarray(mesh.surface_id, dtype=np.int32)
+        self.group_ids = np.unique(groups) if groups.size else np.zeros(0, np.int32)
+        self._sgroup = np.ascontiguousarray(np.searchsorted(self.group_ids, groups), dtype=np.int32)
+        cm = _lib.CmgMesh()
+        cm.n_nodes, cm.n_elements = mesh.n_nodes, E
+        cm.n_surfaces = self._surf.shape[0]
+        cm.nodes_per_element = npe
+        cm.nodes_per_surface = self._surf.shape[1] if cm.n_surfaces else 0
+        cm.n_groups = max(1, len(self.group_ids))
+        cm.ref_coords, cm.elements, cm.bases = (self._coords.ctypes.data, self._elements.ctypes.data,
+                                                self._bases.ctypes.data)
+        cm.material_index, cm.activation_time, cm.bc = (self._matidx.ctypes.data, self._tact.ctypes.data,
+                                                        self._bc.ctypes.data)
+        cm.surfaces = self._surf.ctypes.data if cm.n_surfaces else None
+        cm.surface_group = self._sgroup.ctypes.data if cm.n_surfaces else None
+        self.n_groups = cm.n_groups
+        mats = (_lib.CmgMaterial * len(self.materials))(*[to_cmg_material(m) for m in self.materials])
+        h = ctypes.c_void_p()
+        rc = self.L.cmg_create(ctypes.byref(cm), mats, len(self.materials), ctypes.byref(self.opt), ctypes.byref(h))
+        if rc != _lib.SUCCESS:
+            raise CmgError(rc, "cmg_create")
+        self.h = h
+        n_nodes, n_el, n_dofs, nnzb = (ctypes.c_int64() for _ in range(4))
+        self._check(self.L.cmg_sizes(self.h, ctypes.byref(n_nodes), ctypes.byref(n_el), ctypes.byref(n_dofs),
+                                     ctypes.byref(nnzb)), "cmg_sizes")
+        self.n_nodes, self.n_dofs, self.nnzb = n_nodes.value, n_dofs.value, nnzb.value
+        self.rowptr = np.zeros(self.n_nodes + 1, np.int64)
+        self.colind = np.zeros(self.nnzb, np.int32)
+        self._check(self.L.cmg_pattern(self.h, _ptr(self.rowptr), _ptr(self.colind)), "cmg_pattern")
+
+    # -- plumbing -------------------------------------------------------------
+    def _check(self, rc, where):
+        if rc != _lib.SUCCESS:
+            raise CmgError(rc, where)
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cmg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def value_dtype(self):
+        return np.float32 if self.fp32 else np.float64
+
+    def bsr(self, values) -> sp.bsr_matrix:
+        v = np.asarray(values).reshape(self.nnzb, 3, 3)
+        return sp.bsr_matrix((v, self.colind, self.rowptr), shape=(self.n_dofs, self.n_dofs))
+
+    # -- element loop ------------------------------------------------------------
+    def set_pressures(self, pressures):
+        p = np.ascontiguousarray(np.resize(np.asarray(pressures, np.float64), self.n_groups))
+        self._check(self.L.cmg_set_pressures(self.h, _ptr(p)), "cmg_set_pressures")
+
+    def residual(self, x, time=0.0, include_pressure=True):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        f = np.zeros(self.n_dofs)
+        self._check(self.L.cmg_residual(self.h, _ptr(x), time, int(include_pressure), _ptr(f)), "cmg_residual")
+        return f
+
+    def tangent(self, x, time=0.0, include_pressure=True):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        v = np.zeros(self.nnzb * 9, self.value_dtype)
+        self._check(self.L.cmg_tangent(self.h, _ptr(x), time, int(include_pressure), _ptr(v)), "cmg_tangent")
+        return v
+
+    def element_residuals(self, x, time=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        nd = 3 * self.mesh.npe
+        fe = np.zeros((self.mesh.n_elements, nd))
+        self._check(self.L.cmg_element_residuals(self.h, _ptr(x), time, _ptr(fe)), "cmg_element_residuals")
+        return fe
+
+    def element_tangents(self, x, time=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        nd = 3 * self.mesh.npe
+        Ke = np.zeros((self.mesh.n_elements, nd, nd), self.value_dtype)
+        self._check(self.L.cmg_element_tangents(self.h, _ptr(x), time, _ptr(Ke)), "cmg_element_tangents")
+        return Ke
+
+    def surface_forces(self, x, pressures):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        p = np.ascontiguousarray(np.resize(np.asarray(pressures, np.float64), self.n_groups))
+        f = np.zeros(self.n_dofs)
+        self._check(self.L.cmg_surface_forces(self.h, _ptr(x), _ptr(p), _ptr(f)), "cmg_surface_forces")
+        return f
+
+    def cavity_volumes(self, x=None):
+        v = np.zeros(self.n_groups)
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64).ravel()
+        self._check(self.L.cmg_cavity_volumes(self.h, _ptr(xa), _ptr(v)), "cmg_cavity_volumes")
+        return v
+
+    def mass_matrix(self):
+        v = np.zeros(self.nnzb * 9)
+        self._check(self.L.cmg_mass_matrix(self.h, _ptr(v)), "cmg_mass_matrix")
+        return v
+
+    def spmv(self, values, x):
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n_dofs)
+        self._check(self.L.cmg_spmv(self.h, _ptr(values), _ptr(x), _ptr(y)), "cmg_spmv")
+        return y
+
+    def linear_solve(self, values, b, x0=None, rtol=1e-10):
+        values = np.ascontiguousarray(values, dtype=self.value_dtype)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n_dofs) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+        st = _lib.CmgSolveStats()
+        rc = self.L.cmg_linear_solve(self.h, _ptr(values), _ptr(b), _ptr(x), rtol, ctypes.byref(st))
+        if rc not in (_lib.SUCCESS, _lib.FAILED):
+            raise CmgError(rc, "cmg_linear_solve")
+        return x, st
+
+    # -- solver state / steps -------------------------------------------------------
+    def set_state(self, nodes=None, velocity=None, acceleration=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+                for a in (nodes, velocity, acceleration)]
+        self._check(self.L.cmg_set_state(self.h, *[_ptr(a) for a in arrs]), "cmg_set_state")
+
+    def get_state(self):
+        out = [np.zeros(self.n_dofs) for _ in range(4)]
+        self._check(self.L.cmg_get_state(self.h, *[_ptr(a) for a in out]), "cmg_get_state")
+        return dict(zip(("nodes", "velocity", "acceleration", "increment"), out))
+
+    def set_initial_guess(self, increment):
+        inc = np.ascontiguousarray(increment, dtype=np.float64).ravel()
+        self._check(self.L.cmg_set_initial_guess(self.h, _ptr(inc)), "cmg_set_initial_guess")
+
+    def static_step(self, time, dt):
+        st = _lib.CmgStepStats()
+        rc = self.L.cmg_static_step(self.h, time, dt, ctypes.byref(st))
+        if rc not in (_lib.SUCCESS, _lib.FAILED):
+            raise CmgError(rc, "cmg_static_step")
+        return rc, st
+
+    def newmark_step(self, time, dt):
+        st = _lib.CmgStepStats()
+        rc = self.L.cmg_newmark_step(self.h, time, dt, ctypes.byref(st))
+        if rc not in (_lib.SUCCESS, _lib.FAILED):
+            raise CmgError(rc, "cmg_newmark_step")
+        return rc, st

@@ -1,0 +1,1127 @@
+// Host side of the C ABI (include/cmg.h): context creation (mesh upload, block
+// pattern, deterministic assembly maps), element-loop entry points, GMRES and the
+// static / Newmark Newton drivers.  Runs one HIP stream per context.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "cmg_internal.h"
+
+namespace cmg {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace cmg
+
+using namespace cmg;
+
+// ---------------------------------------------------------------------------
+// device buffer helper
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    bool alloc(size_t b) {
+        release();
+        if (b == 0) return true;
+        if (hipMalloc(&p, b) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        bytes = b;
+        return true;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct cmg_ctx {
+    cmg_options opt;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int npe = 10, nqp = 4, nps = 0;
+    int64_t N = 0, E = 0, S = 0, n = 0;   // nodes, elements, surfaces, dofs
+    int n_groups = 0;
+    bool fp32 = false;
+    // host copies needed for maps / exports
+    std::vector<uint8_t> h_bc;
+    std::vector<int64_t> h_rowptr;
+    std::vector<int32_t> h_colind;
+    std::vector<double> h_mval;
+    std::vector<double> h_pressure;
+    int64_t nnzb = 0;
+    int64_t n_solid_blocks = 0;
+    // element data (SoA)
+    DevBuf conn, dNdX, bases, w, matidx, tact, mats, bcmask;
+    DevBuf fe, Ke;
+    // surfaces
+    DevBuf sconn, sgroup, pressure, fs, Ks, vol_partial;
+    // pattern and maps
+    DevBuf rowptr, colind, blkrow, cptr, contrib, mval;
+    DevBuf inc_ptr, inc, sinc_ptr, sinc;
+    // state and work vectors
+    DevBuf Xref, nodes, xeval, d, dpred, vpred, vel, acc, r, delta, dsave, tmpA, tmpB;
+    DevBuf vals, dinv;
+    DevBuf gm_V, gm_partial, gm_h, gm_y;
+    int gm_m = 0;
+    double* h_pinned = nullptr;   // host staging for reductions
+    DevBuf corrupt;
+    // static solver bookkeeping (CBSolverEquilibrium::UpdateInitialGuess)
+    double time_last = 0.0, step_last = 0.0;
+    DevBuf guess;               // normalised previous increment
+    bool has_override = false;  // cmg_set_initial_guess
+    DevBuf override_guess;
+};
+
+namespace {
+
+int fail(const std::string& m, int code = CMG_ERR_INVALID) {
+    set_error(m);
+    return code;
+}
+
+template <typename T>
+int upload(DevBuf& b, const T* src, size_t count, hipStream_t s) {
+    if (!b.alloc(count * sizeof(T))) return fail("hipMalloc failed", CMG_ERR_NOMEM);
+    if (count) CMG_HIP_CHECK(hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return CMG_SUCCESS;
+}
+
+int alloc(DevBuf& b, size_t bytes) {
+    if (!b.alloc(bytes)) return fail("hipMalloc of " + std::to_string(bytes) + " bytes failed", CMG_ERR_NOMEM);
+    return CMG_SUCCESS;
+}
+
+#define CMG_TRY(expr)                       \
+    do {                                    \
+        int rc__ = (expr);                  \
+        if (rc__ != CMG_SUCCESS) return rc__; \
+    } while (0)
+
+// T4 shape derivatives, CBElementSolidT4::CalcShapeFunctionDerivatives (:73-121)
+double t4_shape(const double* X, double* g) {
+    double z43 = (X[11] - X[8]), z42 = (X[11] - X[5]), z41 = (X[11] - X[2]);
+    double z32 = (X[8] - X[5]), z31 = (X[8] - X[2]), z21 = (X[5] - X[2]);
+    double y43 = (X[10] - X[7]), y42 = (X[10] - X[4]), y41 = (X[10] - X[1]);
+    double y32 = (X[7] - X[4]), y31 = (X[7] - X[1]), y21 = (X[4] - X[1]);
+    double x41 = (X[9] - X[0]), x31 = (X[6] - X[0]), x21 = (X[3] - X[0]);
+    double d = x21 * (y31 * z41 - y41 * z31) + y21 * (x41 * z31 - x31 * z41) + z21 * (x31 * y41 - x41 * y31);
+    g[0] = 1.0 / d * (X[4] * z43 - X[7] * z42 + X[10] * z32);
+    g[3] = 1.0 / d * (-X[1] * z43 + X[7] * z41 - X[10] * z31);
+    g[6] = 1.0 / d * (X[1] * z42 - X[4] * z41 + X[10] * z21);
+    g[9] = 1.0 / d * (-X[1] * z32 + X[4] * z31 - X[7] * z21);
+    g[1] = 1.0 / d * (-X[3] * z43 + X[6] * z42 - X[9] * z32);
+    g[4] = 1.0 / d * (X[0] * z43 - X[6] * z41 + X[9] * z31);
+    g[7] = 1.0 / d * (-X[0] * z42 + X[3] * z41 - X[9] * z21);
+    g[10] = 1.0 / d * (X[0] * z32 - X[3] * z31 + X[6] * z21);
+    g[2] = 1.0 / d * (X[3] * y43 - X[6] * y42 + X[9] * y32);
+    g[5] = 1.0 / d * (-X[0] * y43 + X[6] * y41 - X[9] * y31);
+    g[8] = 1.0 / d * (X[0] * y42 - X[3] * y41 + X[9] * y21);
+    g[11] = 1.0 / d * (-X[0] * y32 + X[3] * y31 - X[6] * y21);
+    return d;
+}
+
+// T10 consistent mass pattern (CBElementSolidT10.cpp:371-380)
+const double kR1[100] = {6,  1,  1,  1,  -4, -6, -4, -4, -6, -6, 1,  6,  1,  1,  -4, -4, -6, -6, -4, -6,
+                         1,  1,  6,  1,  -6, -4, -4, -6, -6, -4, 1,  1,  1,  6,  -6, -6, -6, -4, -4, -4,
+                         -4, -4, -6, -6, 32, 16, 16, 16, 16, 8,  -6, -4, -4, -6, 16, 32, 16, 8,  16, 16,
+                         -4, -6, -4, -6, 16, 16, 32, 16, 8,  16, -4, -6, -6, -4, 16, 8,  16, 32, 16, 16,
+                         -6, -4, -6, -4, 16, 16, 8,  16, 32, 16, -6, -6, -4, -4, 8,  16, 16, 16, 16, 32};
+
+int64_t find_slot(const cmg_ctx* c, int64_t r, int32_t col) {
+    const int32_t* b = c->h_colind.data() + c->h_rowptr[r];
+    const int32_t* e = c->h_colind.data() + c->h_rowptr[r + 1];
+    const int32_t* p = std::lower_bound(b, e, col);
+    if (p == e || *p != col) return -1;
+    return c->h_rowptr[r] + (p - b);
+}
+
+// CSR incidence list node -> (item * width + local) in item order
+void build_incidence(int64_t N, int64_t items, int width, const int32_t* conn, std::vector<int64_t>& ptr,
+                     std::vector<int32_t>& inc) {
+    ptr.assign(N + 1, 0);
+    for (int64_t i = 0; i < items * width; i++) ptr[conn[i] + 1]++;
+    for (int64_t r = 0; r < N; r++) ptr[r + 1] += ptr[r];
+    inc.resize(ptr[N]);
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t i = 0; i < items * width; i++) inc[fill[conn[i]]++] = (int32_t)i;
+}
+
+}  // namespace
+
+// ===========================================================================
+// ABI
+// ===========================================================================
+extern "C" {
+
+void cmg_default_options(cmg_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->precision = CMG_PREC_FP64;
+    o->transpose_tangent = 0;
+    o->pressure_tangent = 1;
+    o->consistent_mass = 1;
+    o->newton_maxit = 50;
+    o->krylov_restart = 80;
+    o->krylov_maxit = 4000;
+    o->device = 0;
+    o->newton_atol = 1e-8;
+    o->newton_rtol = 1e-8;
+    o->newton_stol = 1e-8;
+    o->krylov_rtol = 1e-6;
+    o->newmark_beta = 0.25;
+    o->newmark_gamma = 0.5;
+    o->fd_epsilon = 1e-12;
+}
+
+const char* cmg_last_error(void) { return g_last_error.c_str(); }
+
+int cmg_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void cmg_destroy(cmg_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_materials, const cmg_options* options,
+               cmg_ctx** out) {
+    if (!mesh || !materials || n_materials <= 0 || !out) return fail("cmg_create: null argument");
+    *out = nullptr;
+    const int npe = mesh->nodes_per_element;
+    if (npe != 10 && npe != 4) return fail("cmg_create: nodes_per_element must be 10 or 4");
+    if (mesh->n_nodes <= 0 || mesh->n_elements <= 0 || !mesh->ref_coords || !mesh->elements || !mesh->bases)
+        return fail("cmg_create: empty mesh");
+    if (mesh->n_nodes > INT_MAX - 1) return fail("cmg_create: too many nodes");
+    if (mesh->n_surfaces > 0 && mesh->nodes_per_surface != 3 && mesh->nodes_per_surface != 6)
+        return fail("cmg_create: nodes_per_surface must be 3 or 6");
+    std::unique_ptr<cmg_ctx> c(new cmg_ctx());
+    if (options)
+        c->opt = *options;
+    else
+        cmg_default_options(&c->opt);
+    c->device = c->opt.device;
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    CMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CMG_HIP_CHECK(hipHostMalloc((void**)&c->h_pinned, 4096 * sizeof(double), hipHostMallocDefault));
+    hipStream_t s = c->stream;
+    c->npe = npe;
+    c->nqp = (npe == 10) ? 4 : 1;
+    c->N = mesh->n_nodes;
+    c->E = mesh->n_elements;
+    c->n = 3 * c->N;
+    c->S = mesh->n_surfaces > 0 ? mesh->n_surfaces : 0;
+    c->nps = c->S > 0 ? mesh->nodes_per_surface : 0;
+    c->n_groups = std::max(1, mesh->n_groups);
+    c->fp32 = c->opt.precision == CMG_PREC_MIXED;
+    const int64_t N = c->N, E = c->E;
+    // -- validate connectivity
+    for (int64_t i = 0; i < E * npe; i++)
+        if (mesh->elements[i] < 0 || mesh->elements[i] >= N) return fail("cmg_create: element node id out of range");
+    for (int64_t i = 0; i < c->S * c->nps; i++)
+        if (mesh->surfaces[i] < 0 || mesh->surfaces[i] >= N) return fail("cmg_create: surface node id out of range");
+    if (mesh->material_index)
+        for (int64_t e = 0; e < E; e++)
+            if (mesh->material_index[e] < 0 || mesh->material_index[e] >= n_materials)
+                return fail("cmg_create: material index out of range");
+    if (c->S > 0 && mesh->surface_group)
+        for (int64_t i = 0; i < c->S; i++)
+            if (mesh->surface_group[i] < 0 || mesh->surface_group[i] >= c->n_groups)
+                return fail("cmg_create: surface group out of range");
+    if ((int64_t)E * npe * npe >= ((int64_t)1 << 60)) return fail("cmg_create: mesh too large");
+    // -- materials
+    std::vector<DevMaterial> dm(n_materials);
+    for (int i = 0; i < n_materials; i++) {
+        const cmg_material& m = materials[i];
+        if (m.law != CMG_LAW_GUCCIONE && m.law != CMG_LAW_HOLZAPFEL) return fail("cmg_create: unknown law");
+        DevMaterial& d = dm[i];
+        std::memset(&d, 0, sizeof(d));
+        d.law = m.law;
+        d.ignore_corrupt = m.ignore_corrupt;
+        d.tension_fn = m.tension_fn;
+        d.density = m.density;
+        for (int k = 0; k < 12; k++) d.p[k] = m.params[k];
+        d.tension_max = m.tension_max;
+        for (int k = 0; k < 12; k++) d.fn[k] = m.fn[k];
+        if (d.tension_fn == CMG_FN_DOUBLEHILL && !(d.fn[11] > 0.0)) {
+            double mx = -INFINITY;   // CBAnalyticFunction.h:155-165
+            for (int q = 0; q <= 1000; q++) {
+                double t = q * d.fn[5] / 1000;
+                double cc = std::pow(t / d.fn[6], d.fn[8]);
+                cc = cc / (1.0 + cc);
+                double rr = std::pow(t / d.fn[7], d.fn[9]);
+                rr = 1.0 / (1.0 + rr);
+                mx = std::max(mx, cc * rr);
+            }
+            d.fn[11] = mx;
+        }
+    }
+    CMG_TRY(upload(c->mats, dm.data(), dm.size(), s));
+    // -- per-element SoA data
+    std::vector<int32_t> conn_soa((size_t)npe * E);
+    for (int64_t e = 0; e < E; e++)
+        for (int k = 0; k < npe; k++) conn_soa[(size_t)k * E + e] = mesh->elements[e * npe + k];
+    CMG_TRY(upload(c->conn, conn_soa.data(), conn_soa.size(), s));
+    const int nb_in = (npe == 10) ? 5 : 1;
+    std::vector<double> bases_soa((size_t)9 * c->nqp * E);
+    for (int64_t e = 0; e < E; e++)
+        for (int q = 0; q < c->nqp; q++) {
+            const double* B = mesh->bases + ((size_t)e * nb_in + (npe == 10 ? q + 1 : 0)) * 9;
+            for (int r = 0; r < 9; r++) bases_soa[(size_t)(q * 9 + r) * E + e] = B[r];
+        }
+    CMG_TRY(upload(c->bases, bases_soa.data(), bases_soa.size(), s));
+    std::vector<double>().swap(bases_soa);
+    std::vector<double> tact(E, 0.0);
+    if (mesh->activation_time) std::copy(mesh->activation_time, mesh->activation_time + E, tact.begin());
+    CMG_TRY(upload(c->tact, tact.data(), tact.size(), s));
+    if (mesh->material_index) CMG_TRY(upload(c->matidx, mesh->material_index, E, s));
+    c->h_bc.assign(N, 0);
+    if (mesh->bc)
+        for (int64_t i = 0; i < N; i++) c->h_bc[i] = mesh->bc[i] & 7;
+    CMG_TRY(upload(c->bcmask, c->h_bc.data(), N, s));
+    CMG_TRY(upload(c->Xref, mesh->ref_coords, 3 * N, s));
+    CMG_TRY(alloc(c->w, E * sizeof(double)));
+    CMG_TRY(alloc(c->dNdX, (size_t)c->nqp * npe * 3 * E * sizeof(double)));
+    std::vector<double> wh(E);
+    if (npe == 10) {
+        DevBuf conn_aos;
+        CMG_TRY(upload(conn_aos, mesh->elements, (size_t)E * npe, s));
+        launch_t10_shape(E, conn_aos.as<int32_t>(), c->Xref.as<double>(), c->dNdX.as<double>(), c->w.as<double>(),
+                         s);
+        CMG_HIP_CHECK(hipGetLastError());
+        CMG_HIP_CHECK(hipMemcpyAsync(wh.data(), c->w.p, E * sizeof(double), hipMemcpyDeviceToHost, s));
+        CMG_HIP_CHECK(hipStreamSynchronize(s));
+        for (auto& v : wh) v *= 24.0;   // detJ for the mass matrix
+    } else {
+        std::vector<double> g_soa((size_t)12 * E);
+        std::vector<double> wv(E);
+        for (int64_t e = 0; e < E; e++) {
+            double X[12], g[12];
+            for (int k = 0; k < 4; k++)
+                for (int d = 0; d < 3; d++) X[3 * k + d] = mesh->ref_coords[3 * (int64_t)mesh->elements[e * 4 + k] + d];
+            const double detJ = t4_shape(X, g);
+            for (int r = 0; r < 12; r++) g_soa[(size_t)r * E + e] = g[r];
+            wv[e] = detJ / 6.;   // initialVolume_ = GetVolume() = detJ/6
+            wh[e] = detJ;
+        }
+        CMG_HIP_CHECK(hipMemcpyAsync(c->dNdX.p, g_soa.data(), g_soa.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        CMG_HIP_CHECK(hipMemcpyAsync(c->w.p, wv.data(), E * sizeof(double), hipMemcpyHostToDevice, s));
+        CMG_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    // -- incidence lists
+    std::vector<int64_t> iptr;
+    std::vector<int32_t> inc;
+    build_incidence(N, E, npe, mesh->elements, iptr, inc);
+    CMG_TRY(upload(c->inc_ptr, iptr.data(), iptr.size(), s));
+    CMG_TRY(upload(c->inc, inc.data(), inc.size(), s));
+    std::vector<int64_t> sptr;
+    std::vector<int32_t> sinc;
+    if (c->S > 0) {
+        build_incidence(N, c->S, c->nps, mesh->surfaces, sptr, sinc);
+        CMG_TRY(upload(c->sinc_ptr, sptr.data(), sptr.size(), s));
+        CMG_TRY(upload(c->sinc, sinc.data(), sinc.size(), s));
+    }
+    // -- block pattern: node r couples to every node sharing an element (CBModel.cpp:43-83)
+    {
+        c->h_rowptr.assign(N + 1, 0);
+        std::vector<std::vector<int32_t>> rows;   // built per node then flattened
+        std::vector<int32_t> tmp;
+        std::vector<int32_t> cols;
+        cols.reserve((size_t)N * 30);
+        for (int64_t r = 0; r < N; r++) {
+            tmp.clear();
+            for (int64_t k = iptr[r]; k < iptr[r + 1]; k++) {
+                const int64_t e = inc[k] / npe;
+                for (int b = 0; b < npe; b++) tmp.push_back(mesh->elements[e * npe + b]);
+            }
+            if (tmp.empty()) tmp.push_back((int32_t)r);   // isolated node: keep a diagonal block
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            cols.insert(cols.end(), tmp.begin(), tmp.end());
+            c->h_rowptr[r + 1] = (int64_t)cols.size();
+        }
+        c->h_colind.swap(cols);
+        c->nnzb = (int64_t)c->h_colind.size();
+    }
+    const int64_t nnzb = c->nnzb;
+    std::vector<int32_t> blkrow(nnzb);
+    for (int64_t r = 0; r < N; r++)
+        for (int64_t k = c->h_rowptr[r]; k < c->h_rowptr[r + 1]; k++) blkrow[k] = (int32_t)r;
+    CMG_TRY(upload(c->rowptr, c->h_rowptr.data(), N + 1, s));
+    CMG_TRY(upload(c->colind, c->h_colind.data(), nnzb, s));
+    CMG_TRY(upload(c->blkrow, blkrow.data(), nnzb, s));
+    // -- assembly contributor lists (deterministic order: solid elements, then surfaces)
+    {
+        const int64_t ns_blocks = E * npe * npe;
+        c->n_solid_blocks = ns_blocks;
+        const int nps = c->nps;
+        std::vector<int64_t> slot_of((size_t)ns_blocks);
+        std::vector<int64_t> cnt(nnzb + 1, 0);
+        for (int64_t e = 0; e < E; e++)
+            for (int a = 0; a < npe; a++)
+                for (int b = 0; b < npe; b++) {
+                    const int64_t sl = find_slot(c.get(), mesh->elements[e * npe + a], mesh->elements[e * npe + b]);
+                    if (sl < 0) return fail("cmg_create: pattern construction failed");
+                    slot_of[(e * npe + a) * npe + b] = sl;
+                    cnt[sl + 1]++;
+                }
+        std::vector<int64_t> sslot((size_t)c->S * nps * nps);
+        for (int64_t si = 0; si < c->S; si++)
+            for (int a = 0; a < nps; a++)
+                for (int b = 0; b < nps; b++) {
+                    const int64_t sl = find_slot(c.get(), mesh->surfaces[si * nps + a], mesh->surfaces[si * nps + b]);
+                    if (sl < 0) return fail("cmg_create: surface element nodes do not share a solid element");
+                    sslot[(si * nps + a) * nps + b] = sl;
+                    cnt[sl + 1]++;
+                }
+        for (int64_t k = 0; k < nnzb; k++) cnt[k + 1] += cnt[k];
+        std::vector<int64_t> contrib(cnt[nnzb]);
+        std::vector<int64_t> fillp(cnt.begin(), cnt.end() - 1);
+        const bool trans = c->opt.transpose_tangent != 0;
+        for (int64_t e = 0; e < E; e++)
+            for (int a = 0; a < npe; a++)
+                for (int b = 0; b < npe; b++) {
+                    const int64_t sl = slot_of[(e * npe + a) * npe + b];
+                    // transposed: block (a,b) of the assembled matrix takes (K_e(b,a))^T
+                    contrib[fillp[sl]++] = trans ? ~((e * npe + b) * npe + a) : ((e * npe + a) * npe + b);
+                }
+        for (int64_t si = 0; si < c->S; si++)
+            for (int a = 0; a < nps; a++)
+                for (int b = 0; b < nps; b++) {
+                    const int64_t sl = sslot[(si * nps + a) * nps + b];
+                    contrib[fillp[sl]++] = ns_blocks + ((((si * nps + a) * nps + b) << 3) | nps);
+                }
+        CMG_TRY(upload(c->cptr, cnt.data(), cnt.size(), s));
+        CMG_TRY(upload(c->contrib, contrib.data(), contrib.size(), s));
+        // -- mass matrix: one scalar per block (consistent CBElementSolidT10.cpp:352-394 /
+        //    CBElementSolidT4.cpp:365-410, or lumped :396-422 / :432-456)
+        c->h_mval.assign(nnzb, 0.0);
+        for (int64_t e = 0; e < E; e++) {
+            const int mi = mesh->material_index ? mesh->material_index[e] : 0;
+            const double rho = dm[mi].density;
+            for (int a = 0; a < npe; a++)
+                for (int b = 0; b < npe; b++) {
+                    double v;
+                    if (npe == 10)
+                        v = c->opt.consistent_mass ? kR1[10 * a + b] * ((rho * wh[e]) / 2520)
+                                                   : ((a == b) ? (rho * (wh[e] / 6)) / 10 : 0.0);
+                    else
+                        v = c->opt.consistent_mass ? ((a == b) ? 2.0 : 1.0) * ((rho * (wh[e] / 6.)) / 20)
+                                                   : ((a == b) ? (rho * (wh[e] / 6.)) / 4 : 0.0);
+                    c->h_mval[slot_of[(e * npe + a) * npe + b]] += v;
+                }
+        }
+        CMG_TRY(upload(c->mval, c->h_mval.data(), nnzb, s));
+    }
+    // -- surfaces
+    c->h_pressure.assign(c->n_groups, 0.0);
+    CMG_TRY(upload(c->pressure, c->h_pressure.data(), c->n_groups, s));
+    if (c->S > 0) {
+        CMG_TRY(upload(c->sconn, mesh->surfaces, (size_t)c->S * c->nps, s));
+        std::vector<int32_t> grp(c->S, 0);
+        if (mesh->surface_group) std::copy(mesh->surface_group, mesh->surface_group + c->S, grp.begin());
+        CMG_TRY(upload(c->sgroup, grp.data(), grp.size(), s));
+        const int ns = 3 * c->nps;
+        CMG_TRY(alloc(c->fs, (size_t)c->S * ns * sizeof(double)));
+        CMG_TRY(alloc(c->Ks, (size_t)c->S * ns * ns * sizeof(double)));
+        CMG_TRY(alloc(c->vol_partial, (size_t)c->S * sizeof(double)));
+    }
+    // -- work buffers
+    const size_t vb = (size_t)c->n * sizeof(double);
+    for (DevBuf* b : {&c->nodes, &c->xeval, &c->d, &c->dpred, &c->vpred, &c->vel, &c->acc, &c->r, &c->delta,
+                      &c->dsave, &c->tmpA, &c->tmpB, &c->guess, &c->override_guess})
+        CMG_TRY(alloc(*b, vb));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->nodes.p, mesh->ref_coords, vb, hipMemcpyHostToDevice, s));
+    for (DevBuf* b : {&c->d, &c->vel, &c->acc, &c->guess, &c->dpred, &c->vpred})
+        launch_fill(c->n, 0.0, b->as<double>(), s);
+    CMG_TRY(alloc(c->fe, (size_t)E * 3 * npe * sizeof(double)));
+    CMG_TRY(alloc(c->Ke, (size_t)E * 9 * npe * npe * (c->fp32 ? sizeof(float) : sizeof(double))));
+    CMG_TRY(alloc(c->vals, (size_t)nnzb * 9 * (c->fp32 ? sizeof(float) : sizeof(double))));
+    CMG_TRY(alloc(c->dinv, (size_t)N * 9 * sizeof(double)));
+    CMG_TRY(alloc(c->corrupt, sizeof(int)));
+    c->gm_m = std::max(2, std::min(c->opt.krylov_restart, 1000));
+    CMG_TRY(alloc(c->gm_V, (size_t)(c->gm_m + 2) * vb));
+    CMG_TRY(alloc(c->gm_partial, (size_t)dot_blocks(c->n) * (c->gm_m + 2) * sizeof(double) + 64));
+    CMG_TRY(alloc(c->gm_h, (size_t)(c->gm_m + 2) * sizeof(double)));
+    CMG_TRY(alloc(c->gm_y, (size_t)(c->gm_m + 2) * sizeof(double)));
+    CMG_HIP_CHECK(hipGetLastError());
+    CMG_HIP_CHECK(hipStreamSynchronize(s));
+    *out = c.release();
+    return CMG_SUCCESS;
+}
+
+int cmg_sizes(cmg_ctx* c, int64_t* n_nodes, int64_t* n_elements, int64_t* n_dofs, int64_t* nnz_blocks) {
+    if (!c) return fail("null context");
+    if (n_nodes) *n_nodes = c->N;
+    if (n_elements) *n_elements = c->E;
+    if (n_dofs) *n_dofs = c->n;
+    if (nnz_blocks) *nnz_blocks = c->nnzb;
+    return CMG_SUCCESS;
+}
+
+int cmg_pattern(cmg_ctx* c, int64_t* rowptr, int32_t* colind) {
+    if (!c) return fail("null context");
+    if (rowptr) std::copy(c->h_rowptr.begin(), c->h_rowptr.end(), rowptr);
+    if (colind) std::copy(c->h_colind.begin(), c->h_colind.end(), colind);
+    return CMG_SUCCESS;
+}
+
+void* cmg_stream(cmg_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int cmg_synchronize(cmg_ctx* c) {
+    if (!c) return fail("null context");
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_set_pressures(cmg_ctx* c, const double* p) {
+    if (!c || !p) return fail("cmg_set_pressures: null argument");
+    std::copy(p, p + c->n_groups, c->h_pressure.begin());
+    CMG_HIP_CHECK(hipMemcpyAsync(c->pressure.p, c->h_pressure.data(), c->n_groups * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// internal evaluation helpers
+// ===========================================================================
+namespace {
+
+ElemArgs elem_args(cmg_ctx* c, const double* x_dev, double time) {
+    ElemArgs a;
+    a.E = c->E;
+    a.conn = c->conn.as<int32_t>();
+    a.dNdX = c->dNdX.as<double>();
+    a.bases = c->bases.as<double>();
+    a.w = c->w.as<double>();
+    a.matidx = c->matidx.p ? c->matidx.as<int32_t>() : nullptr;
+    a.tact = c->tact.as<double>();
+    a.mats = c->mats.as<DevMaterial>();
+    a.x = x_dev;
+    a.time = time;
+    a.fe = c->fe.as<double>();
+    a.Ke = c->Ke.p;
+    a.corrupt = c->corrupt.as<int>();
+    return a;
+}
+
+SurfArgs surf_args(cmg_ctx* c, const double* x_dev, bool with_volume) {
+    SurfArgs a;
+    a.S = c->S;
+    a.nps = c->nps;
+    a.conn = c->sconn.as<int32_t>();
+    a.group = c->sgroup.as<int32_t>();
+    a.pressure = c->pressure.as<double>();
+    a.bcmask = c->bcmask.as<uint8_t>();
+    a.x = x_dev;
+    a.fs = c->fs.as<double>();
+    a.Ks = c->Ks.as<double>();
+    a.vol_partial = with_volume ? c->vol_partial.as<double>() : nullptr;
+    return a;
+}
+
+// element kernels (+ surfaces) at x_dev; returns CMG_SUCCESS or CMG_CORRUPT_ELEMENT
+int eval_elements(cmg_ctx* c, const double* x_dev, double time, bool tangent, bool pressure, int* corrupt_id) {
+    hipStream_t s = c->stream;
+    launch_fill_int(1, INT_MAX, c->corrupt.as<int>(), s);
+    ElemArgs a = elem_args(c, x_dev, time);
+    if (tangent)
+        launch_element_residual_tangent(c->npe, c->fp32, a, s);
+    else
+        launch_element_residual(c->npe, a, s);
+    if (pressure && c->S > 0) launch_surface(surf_args(c, x_dev, false), tangent && c->opt.pressure_tangent, s);
+    CMG_HIP_CHECK(hipGetLastError());
+    int flag = INT_MAX;
+    CMG_HIP_CHECK(hipMemcpyAsync(c->h_pinned, c->corrupt.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    CMG_HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(&flag, c->h_pinned, sizeof(int));
+    if (corrupt_id) *corrupt_id = (flag == INT_MAX) ? -1 : flag;
+    if (flag != INT_MAX) {
+        set_error("element " + std::to_string(flag) + " is corrupt (det F <= 0)");
+        return CMG_CORRUPT_ELEMENT;
+    }
+    return CMG_SUCCESS;
+}
+
+void gather_residual(cmg_ctx* c, bool pressure, double* f_dev) {
+    const bool sp = pressure && c->S > 0;
+    launch_gather_vector(c->N, c->inc_ptr.as<int64_t>(), c->inc.as<int32_t>(), c->fe.as<double>(), 3 * c->npe,
+                         sp ? c->sinc_ptr.as<int64_t>() : nullptr, sp ? c->sinc.as<int32_t>() : nullptr,
+                         sp ? c->fs.as<double>() : nullptr, 3 * c->nps, c->bcmask.as<uint8_t>(), f_dev, c->stream);
+}
+
+void gather_tangent(cmg_ctx* c, bool pressure, double cm, void* vals_dev) {
+    const bool sp = pressure && c->S > 0 && c->opt.pressure_tangent;
+    // without a pressure tangent the surface blocks read zeros: clear Ks
+    if (c->S > 0 && !sp) launch_fill((int64_t)c->S * 9 * c->nps * c->nps, 0.0, c->Ks.as<double>(), c->stream);
+    launch_gather_matrix(c->nnzb, c->cptr.as<int64_t>(), c->contrib.as<int64_t>(), c->Ke.p, c->fp32,
+                         c->Ks.as<double>(), c->n_solid_blocks, c->blkrow.as<int32_t>(), c->colind.as<int32_t>(),
+                         cm != 0.0 ? c->mval.as<double>() : nullptr, cm, c->bcmask.as<uint8_t>(), vals_dev, c->fp32,
+                         c->stream);
+}
+
+// ||v||_2 (synchronising)
+int norm2(cmg_ctx* c, const double* v, double* out) {
+    hipStream_t s = c->stream;
+    const int nb = dot_blocks(c->n);
+    launch_multidot(c->n, 1, v, 0, v, c->gm_partial.as<double>(), s);
+    launch_reduce_partials(nb, 1, c->gm_partial.as<double>(), c->gm_h.as<double>(), s);
+    CMG_HIP_CHECK(hipMemcpyAsync(c->h_pinned, c->gm_h.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    CMG_HIP_CHECK(hipStreamSynchronize(s));
+    *out = std::sqrt(std::max(0.0, c->h_pinned[0]));
+    return CMG_SUCCESS;
+}
+
+// dots of V_0..V_{k-1} (stride ld) with w -> host array h
+int multidot(cmg_ctx* c, int k, const double* V, int64_t ld, const double* w, double* h) {
+    hipStream_t s = c->stream;
+    const int nb = dot_blocks(c->n);
+    launch_multidot(c->n, k, V, ld, w, c->gm_partial.as<double>(), s);
+    launch_reduce_partials(nb, k, c->gm_partial.as<double>(), c->gm_h.as<double>(), s);
+    CMG_HIP_CHECK(hipMemcpyAsync(c->h_pinned, c->gm_h.p, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    CMG_HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(h, c->h_pinned, k * sizeof(double));
+    return CMG_SUCCESS;
+}
+
+int to_device_coeffs(cmg_ctx* c, const double* h, int k) {
+    std::memcpy(c->h_pinned + 2048, h, k * sizeof(double));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->gm_y.p, c->h_pinned + 2048, k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // the pinned staging area is reused by the next call only after a later synchronisation
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+// GMRES(m) with right block-Jacobi preconditioning: solves A x = b (x holds the
+// initial guess).  Classical Gram-Schmidt with DGKS re-orthogonalisation.
+int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol, int maxit, cmg_solve_stats* st) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    const int m = c->gm_m;
+    double* V = c->gm_V.as<double>();
+    double* w = c->tmpA.as<double>();
+    double* z = c->tmpB.as<double>();
+    launch_block_jacobi_setup(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32,
+                              c->dinv.as<double>(), s);
+    double bnorm;
+    CMG_TRY(norm2(c, b, &bnorm));
+    // r = b - A x
+    launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s);
+    launch_waxpby(n, 1.0, b, -1.0, w, V, s);
+    double beta;
+    CMG_TRY(norm2(c, V, &beta));
+    st->initial_residual = beta;
+    st->iterations = 0;
+    st->converged = 0;
+    const double target = rtol * (bnorm > 0 ? bnorm : 1.0);
+    if (beta <= target || beta == 0.0) {
+        st->final_residual = beta;
+        st->converged = 1;
+        return CMG_SUCCESS;
+    }
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), h(m + 2), h2(m + 2), y(m);
+    int total = 0;
+    while (true) {
+        launch_scale_copy(n, 1.0 / beta, V, V, s);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        double resid = beta;
+        for (; j < m && total < maxit;) {
+            double* vj = V + (int64_t)j * n;
+            double* vn = V + (int64_t)(j + 1) * n;
+            launch_block_jacobi_apply(c->N, c->dinv.as<double>(), vj, z, s);
+            launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, z, vn, s);
+            // pass 1: h_i = v_i . w (i <= j) and ||w||^2 in one multi-dot
+            CMG_TRY(multidot(c, j + 2, V, n, vn, h.data()));
+            const double wn2 = h[j + 1];
+            CMG_TRY(to_device_coeffs(c, h.data(), j + 1));
+            launch_multiaxpy(n, j + 1, V, n, c->gm_y.as<double>(), vn, s);
+            double hs = 0.0;
+            for (int i = 0; i <= j; i++) hs += h[i] * h[i];
+            double nrm2 = wn2 - hs;
+            if (nrm2 < 0.5 * wn2) {
+                // strong cancellation: second classical Gram-Schmidt pass
+                CMG_TRY(multidot(c, j + 2, V, n, vn, h2.data()));
+                CMG_TRY(to_device_coeffs(c, h2.data(), j + 1));
+                launch_multiaxpy(n, j + 1, V, n, c->gm_y.as<double>(), vn, s);
+                double hs2 = 0.0;
+                for (int i = 0; i <= j; i++) {
+                    h[i] += h2[i];
+                    hs2 += h2[i] * h2[i];
+                }
+                nrm2 = h2[j + 1] - hs2;
+            }
+            const double hn = std::sqrt(std::max(nrm2, 0.0));
+            for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h[i];
+            H[(size_t)(j + 1) * m + j] = hn;
+            for (int i = 0; i < j; i++) {   // apply previous rotations
+                const double a0 = H[(size_t)i * m + j], a1 = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a0 + sn[i] * a1;
+                H[(size_t)(i + 1) * m + j] = -sn[i] * a0 + cs[i] * a1;
+            }
+            const double a0 = H[(size_t)j * m + j], a1 = H[(size_t)(j + 1) * m + j];
+            const double den = std::hypot(a0, a1);
+            cs[j] = den > 0 ? a0 / den : 1.0;
+            sn[j] = den > 0 ? a1 / den : 0.0;
+            H[(size_t)j * m + j] = den;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            resid = std::fabs(g[j + 1]);
+            total++;
+            j++;
+            if (hn > 0.0) launch_scale_copy(n, 1.0 / hn, vn, vn, s);
+            if (resid <= target || hn == 0.0) break;
+        }
+        // y = H^-1 g, x += M^-1 (V y)
+        for (int i = j - 1; i >= 0; i--) {
+            double sum = g[i];
+            for (int k = i + 1; k < j; k++) sum -= H[(size_t)i * m + k] * y[k];
+            y[i] = sum / H[(size_t)i * m + i];
+        }
+        std::vector<double> ny(j);
+        for (int i = 0; i < j; i++) ny[i] = -y[i];
+        launch_fill(n, 0.0, w, s);
+        CMG_TRY(to_device_coeffs(c, ny.data(), j));
+        launch_multiaxpy(n, j, V, n, c->gm_y.as<double>(), w, s);   // w = V y
+        launch_block_jacobi_apply(c->N, c->dinv.as<double>(), w, z, s);
+        launch_axpby(n, 1.0, z, 1.0, x, s);
+        // true residual
+        launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s);
+        launch_waxpby(n, 1.0, b, -1.0, w, V, s);
+        CMG_TRY(norm2(c, V, &beta));
+        st->iterations = total;
+        st->final_residual = beta;
+        if (beta <= target) {
+            st->converged = 1;
+            break;
+        }
+        if (total >= maxit) break;
+    }
+    CMG_HIP_CHECK(hipGetLastError());
+    return CMG_SUCCESS;
+}
+
+}  // namespace
+
+// ===========================================================================
+// element-loop entry points
+// ===========================================================================
+extern "C" {
+
+int cmg_residual_device(cmg_ctx* c, const double* x_dev, double time, int32_t include_pressure, double* f_dev) {
+    if (!c || !x_dev || !f_dev) return fail("cmg_residual_device: null argument");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    int rc = eval_elements(c, x_dev, time, false, include_pressure != 0, nullptr);
+    if (rc != CMG_SUCCESS) return rc;
+    gather_residual(c, include_pressure != 0, f_dev);
+    CMG_HIP_CHECK(hipGetLastError());
+    return CMG_SUCCESS;
+}
+
+int cmg_residual_tangent_device(cmg_ctx* c, const double* x_dev, double time, int32_t include_pressure, double* f_dev,
+                                void* values_dev) {
+    if (!c || !x_dev) return fail("cmg_residual_tangent_device: null argument");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    int rc = eval_elements(c, x_dev, time, true, include_pressure != 0, nullptr);
+    if (rc != CMG_SUCCESS) return rc;
+    if (f_dev) gather_residual(c, include_pressure != 0, f_dev);
+    if (values_dev) gather_tangent(c, include_pressure != 0, 0.0, values_dev);
+    CMG_HIP_CHECK(hipGetLastError());
+    return CMG_SUCCESS;
+}
+
+int cmg_element_kernel_device(cmg_ctx* c, const double* x_dev, double time, int32_t with_tangent) {
+    if (!c || !x_dev) return fail("cmg_element_kernel_device: null argument");
+    ElemArgs a = elem_args(c, x_dev, time);
+    if (with_tangent)
+        launch_element_residual_tangent(c->npe, c->fp32, a, c->stream);
+    else
+        launch_element_residual(c->npe, a, c->stream);
+    CMG_HIP_CHECK(hipGetLastError());
+    return CMG_SUCCESS;
+}
+
+int cmg_residual(cmg_ctx* c, const double* x, double time, int32_t include_pressure, double* f) {
+    if (!c || !x || !f) return fail("cmg_residual: null argument");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    const size_t vb = c->n * sizeof(double);
+    CMG_HIP_CHECK(hipMemcpyAsync(c->xeval.p, x, vb, hipMemcpyHostToDevice, c->stream));
+    int rc = cmg_residual_device(c, c->xeval.as<double>(), time, include_pressure, c->r.as<double>());
+    if (rc != CMG_SUCCESS) return rc;
+    CMG_HIP_CHECK(hipMemcpyAsync(f, c->r.p, vb, hipMemcpyDeviceToHost, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_tangent(cmg_ctx* c, const double* x, double time, int32_t include_pressure, void* values) {
+    if (!c || !x || !values) return fail("cmg_tangent: null argument");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->xeval.p, x, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = cmg_residual_tangent_device(c, c->xeval.as<double>(), time, include_pressure, nullptr, c->vals.p);
+    if (rc != CMG_SUCCESS) return rc;
+    const size_t vbytes = (size_t)c->nnzb * 9 * (c->fp32 ? sizeof(float) : sizeof(double));
+    CMG_HIP_CHECK(hipMemcpyAsync(values, c->vals.p, vbytes, hipMemcpyDeviceToHost, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_element_residuals(cmg_ctx* c, const double* x, double time, double* fe) {
+    if (!c || !x || !fe) return fail("cmg_element_residuals: null argument");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->xeval.p, x, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = eval_elements(c, c->xeval.as<double>(), time, false, false, nullptr);
+    if (rc != CMG_SUCCESS) return rc;
+    CMG_HIP_CHECK(hipMemcpyAsync(fe, c->fe.p, (size_t)c->E * 3 * c->npe * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_element_tangents(cmg_ctx* c, const double* x, double time, void* Ke) {
+    if (!c || !x || !Ke) return fail("cmg_element_tangents: null argument");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->xeval.p, x, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = eval_elements(c, c->xeval.as<double>(), time, true, false, nullptr);
+    if (rc != CMG_SUCCESS) return rc;
+    const int npe = c->npe, nd = 3 * npe;
+    const size_t es = c->fp32 ? sizeof(float) : sizeof(double);
+    std::vector<char> blk((size_t)c->E * 9 * npe * npe * es);
+    CMG_HIP_CHECK(hipMemcpyAsync(blk.data(), c->Ke.p, blk.size(), hipMemcpyDeviceToHost, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    // block layout [e][a][b][i][j] -> Jacobian layout [e][3a+i][3b+j]
+    for (int64_t e = 0; e < c->E; e++)
+        for (int a = 0; a < npe; a++)
+            for (int b = 0; b < npe; b++)
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        const size_t src = (size_t)e * 9 * npe * npe + (size_t)(a * npe + b) * 9 + 3 * i + j;
+                        const size_t dst = (size_t)e * nd * nd + (size_t)(3 * a + i) * nd + 3 * b + j;
+                        std::memcpy(static_cast<char*>(Ke) + dst * es, blk.data() + src * es, es);
+                    }
+    return CMG_SUCCESS;
+}
+
+int cmg_surface_forces(cmg_ctx* c, const double* x, const double* pressures, double* f) {
+    if (!c || !x || !f) return fail("cmg_surface_forces: null argument");
+    if (pressures) CMG_TRY(cmg_set_pressures(c, pressures));
+    const size_t vb = c->n * sizeof(double);
+    CMG_HIP_CHECK(hipMemcpyAsync(c->xeval.p, x, vb, hipMemcpyHostToDevice, c->stream));
+    launch_fill((int64_t)c->E * 3 * c->npe, 0.0, c->fe.as<double>(), c->stream);
+    if (c->S > 0) launch_surface(surf_args(c, c->xeval.as<double>(), false), false, c->stream);
+    gather_residual(c, true, c->r.as<double>());
+    CMG_HIP_CHECK(hipMemcpyAsync(f, c->r.p, vb, hipMemcpyDeviceToHost, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_cavity_volumes(cmg_ctx* c, const double* x, double* volumes) {
+    if (!c || !volumes) return fail("cmg_cavity_volumes: null argument");
+    const double* xd = c->nodes.as<double>();
+    if (x) {
+        CMG_HIP_CHECK(hipMemcpyAsync(c->xeval.p, x, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        xd = c->xeval.as<double>();
+    }
+    std::vector<double> vp(c->S);
+    if (c->S > 0) {
+        launch_surface(surf_args(c, xd, true), false, c->stream);
+        CMG_HIP_CHECK(hipMemcpyAsync(vp.data(), c->vol_partial.p, c->S * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+    }
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    // summed in surface order on the host: CBCirculationCavity::CalcVolume (:77-94)
+    std::vector<int32_t> grp(c->S, 0);
+    if (c->S > 0)
+        CMG_HIP_CHECK(hipMemcpy(grp.data(), c->sgroup.p, c->S * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int g = 0; g < c->n_groups; g++) volumes[g] = 0.0;
+    for (int64_t i = 0; i < c->S; i++) volumes[grp[i]] += vp[i];
+    return CMG_SUCCESS;
+}
+
+int cmg_mass_matrix(cmg_ctx* c, double* values) {
+    if (!c || !values) return fail("cmg_mass_matrix: null argument");
+    // rows of constrained components are zero (CBElementSolidT10.cpp:386-389)
+    for (int64_t r = 0; r < c->N; r++)
+        for (int64_t k = c->h_rowptr[r]; k < c->h_rowptr[r + 1]; k++)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    values[9 * k + 3 * i + j] = (i == j && !((c->h_bc[r] >> i) & 1)) ? c->h_mval[k] : 0.0;
+    return CMG_SUCCESS;
+}
+
+int cmg_spmv(cmg_ctx* c, const double* values, const double* x, double* y) {
+    if (!c || !values || !x || !y) return fail("cmg_spmv: null argument");
+    const size_t vb = c->n * sizeof(double);
+    DevBuf v64;
+    CMG_TRY(upload(v64, values, (size_t)c->nnzb * 9, c->stream));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->tmpA.p, x, vb, hipMemcpyHostToDevice, c->stream));
+    launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), v64.p, false, c->tmpA.as<double>(),
+                c->tmpB.as<double>(), c->stream);
+    CMG_HIP_CHECK(hipMemcpyAsync(y, c->tmpB.p, vb, hipMemcpyDeviceToHost, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_linear_solve(cmg_ctx* c, const double* values, const double* b, double* x, double rtol, cmg_solve_stats* st) {
+    if (!c || !values || !b || !x) return fail("cmg_linear_solve: null argument");
+    const size_t vb = c->n * sizeof(double);
+    const size_t vbytes = (size_t)c->nnzb * 9 * (c->fp32 ? sizeof(float) : sizeof(double));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->vals.p, values, vbytes, hipMemcpyHostToDevice, c->stream));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->r.p, b, vb, hipMemcpyHostToDevice, c->stream));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->delta.p, x, vb, hipMemcpyHostToDevice, c->stream));
+    cmg_solve_stats local;
+    if (!st) st = &local;
+    CMG_TRY(gmres(c, c->vals.p, c->r.as<double>(), c->delta.as<double>(), rtol, c->opt.krylov_maxit, st));
+    CMG_HIP_CHECK(hipMemcpyAsync(x, c->delta.p, vb, hipMemcpyDeviceToHost, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return st->converged ? CMG_SUCCESS : CMG_FAILED;
+}
+
+// ===========================================================================
+// solver steps
+// ===========================================================================
+int cmg_set_state(cmg_ctx* c, const double* nodes, const double* vel, const double* acc) {
+    if (!c) return fail("null context");
+    const size_t vb = c->n * sizeof(double);
+    if (nodes) CMG_HIP_CHECK(hipMemcpyAsync(c->nodes.p, nodes, vb, hipMemcpyHostToDevice, c->stream));
+    if (vel) CMG_HIP_CHECK(hipMemcpyAsync(c->vel.p, vel, vb, hipMemcpyHostToDevice, c->stream));
+    if (acc) CMG_HIP_CHECK(hipMemcpyAsync(c->acc.p, acc, vb, hipMemcpyHostToDevice, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_get_state(cmg_ctx* c, double* nodes, double* vel, double* acc, double* disp) {
+    if (!c) return fail("null context");
+    const size_t vb = c->n * sizeof(double);
+    if (nodes) CMG_HIP_CHECK(hipMemcpyAsync(nodes, c->nodes.p, vb, hipMemcpyDeviceToHost, c->stream));
+    if (vel) CMG_HIP_CHECK(hipMemcpyAsync(vel, c->vel.p, vb, hipMemcpyDeviceToHost, c->stream));
+    if (acc) CMG_HIP_CHECK(hipMemcpyAsync(acc, c->acc.p, vb, hipMemcpyDeviceToHost, c->stream));
+    if (disp) CMG_HIP_CHECK(hipMemcpyAsync(disp, c->d.p, vb, hipMemcpyDeviceToHost, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_set_initial_guess(cmg_ctx* c, const double* inc) {
+    if (!c || !inc) return fail("cmg_set_initial_guess: null argument");
+    CMG_HIP_CHECK(hipMemcpyAsync(c->override_guess.p, inc, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->has_override = true;
+    return CMG_SUCCESS;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Newton on r(d) = f_int(nodes + d) + f_p(nodes + d) [+ cm M (d - d~)] = 0, the
+// SNES newtonls loop of CBSolverNewmarkBeta.cpp:890 / CBSolverEquilibrium.cpp:287 with
+// convergence on abstol / rtol / stol = Solver.Precision, a fresh exact tangent
+// per iteration, an inexact GMRES solve (Eisenstat-Walker forcing) and
+// backtracking on the residual norm.
+int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    double* d = c->d.as<double>();
+    double* r = c->r.as<double>();
+    double* delta = c->delta.as<double>();
+    double* xe = c->xeval.as<double>();
+    const double t0 = now_s();
+    double t_tan = 0.0, t_kry = 0.0;
+    double r0 = 0.0, rprev = 0.0, rnorm = 0.0, snorm = 0.0, dnorm = 0.0;
+    int kry_total = 0;
+    st->converged_reason = 0;
+    st->corrupt_element = -1;
+    double eta = 1e-2;
+    double lambda = 1.0;
+    int it = 0;
+    for (;; it++) {
+        const double ta = now_s();
+        launch_waxpby(n, 1.0, c->nodes.as<double>(), 1.0, d, xe, s);
+        int corrupt = -1;
+        int rc = eval_elements(c, xe, time, true, true, &corrupt);
+        if (rc != CMG_SUCCESS) {
+            st->corrupt_element = corrupt;
+            if (it > 0 && lambda > 1.0 / 64) {
+                // step into an inverted state: backtrack (SNESSetFunctionDomainError analogue)
+                lambda *= 0.5;
+                launch_axpby(n, -lambda, delta, 1.0, d, s);
+                it--;
+                continue;
+            }
+            st->converged_reason = -6;
+            st->newton_iterations = it;
+            st->krylov_iterations = kry_total;
+            return CMG_FAILED;
+        }
+        gather_residual(c, true, r);
+        if (cm != 0.0)
+            launch_add_mass_term(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), c->mval.as<double>(), cm, d,
+                                 c->dpred.as<double>(), c->bcmask.as<uint8_t>(), r, s);
+        CMG_TRY(norm2(c, r, &rnorm));
+        t_tan += now_s() - ta;
+        if (!std::isfinite(rnorm)) {
+            st->converged_reason = -4;
+            break;
+        }
+        if (it == 0) {
+            r0 = rnorm;
+            st->initial_residual_norm = r0;
+        } else if (rnorm > rprev && lambda > 1.0 / 64 && rnorm > c->opt.newton_atol) {
+            // no decrease: backtrack along the last Newton direction
+            lambda *= 0.5;
+            launch_axpby(n, -lambda, delta, 1.0, d, s);
+            it--;
+            continue;
+        }
+        if (rnorm < c->opt.newton_atol) {
+            st->converged_reason = 2;
+            break;
+        }
+        if (it > 0 && rnorm < c->opt.newton_rtol * r0) {
+            st->converged_reason = 3;
+            break;
+        }
+        if (it > 0 && snorm * lambda < c->opt.newton_stol * dnorm) {
+            st->converged_reason = 4;
+            break;
+        }
+        if (it >= c->opt.newton_maxit) {
+            st->converged_reason = -5;   // SNES_DIVERGED_MAX_IT
+            break;
+        }
+        if (it > 0) eta = std::min(1e-1, std::max(c->opt.krylov_rtol, 0.9 * (rnorm / rprev) * (rnorm / rprev)));
+        rprev = rnorm;
+        const double tb = now_s();
+        gather_tangent(c, true, cm, c->vals.p);
+        t_tan += now_s() - tb;
+        const double tk = now_s();
+        launch_scale_copy(n, -1.0, r, c->dsave.as<double>(), s);
+        launch_fill(n, 0.0, delta, s);
+        cmg_solve_stats ls;
+        CMG_TRY(gmres(c, c->vals.p, c->dsave.as<double>(), delta, eta, c->opt.krylov_maxit, &ls));
+        kry_total += ls.iterations;
+        t_kry += now_s() - tk;
+        lambda = 1.0;
+        launch_axpby(n, 1.0, delta, 1.0, d, s);
+        CMG_TRY(norm2(c, delta, &snorm));
+        CMG_TRY(norm2(c, d, &dnorm));
+    }
+    st->newton_iterations = it;
+    st->krylov_iterations = kry_total;
+    st->residual_norm = rnorm;
+    st->step_norm = snorm;
+    st->tangent_seconds = t_tan;
+    st->krylov_seconds = t_kry;
+    st->wall_seconds = now_s() - t0;
+    return st->converged_reason > 0 ? CMG_SUCCESS : CMG_FAILED;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cmg_static_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
+    if (!c) return fail("null context");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    cmg_step_stats local;
+    cmg_step_stats* st = stats ? stats : &local;
+    std::memset(st, 0, sizeof(*st));
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    double* d = c->d.as<double>();
+    // CBSolverEquilibrium::UpdateInitialGuess (:234-263)
+    if (time > c->time_last) {
+        if (c->step_last > 0.0) {
+            launch_scale_copy(n, 1. / c->step_last, d, c->guess.as<double>(), s);
+            launch_scale_copy(n, 1. * dt / c->step_last, d, d, s);
+        } else {
+            launch_fill(n, 0.0, d, s);
+        }
+    } else if (time == c->time_last) {
+        launch_fill(n, 0.0, d, s);
+    } else {
+        launch_scale_copy(n, dt, c->guess.as<double>(), d, s);
+    }
+    c->time_last = time;
+    c->step_last = dt;
+    if (c->has_override) {
+        launch_scale_copy(n, 1.0, c->override_guess.as<double>(), d, s);
+        c->has_override = false;
+    }
+    launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
+    int rc = newton(c, time, 0.0, st);
+    if (rc != CMG_SUCCESS) {
+        launch_fill(n, 0.0, d, s);   // :303-306
+        CMG_HIP_CHECK(hipStreamSynchronize(s));
+        return CMG_FAILED;
+    }
+    launch_axpby(n, 1.0, d, 1.0, c->nodes.as<double>(), s);   // :359
+    CMG_HIP_CHECK(hipStreamSynchronize(s));
+    return CMG_SUCCESS;
+}
+
+int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
+    if (!c) return fail("null context");
+    if (!(dt > 0.0)) return fail("cmg_newmark_step: dt must be positive");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    cmg_step_stats local;
+    cmg_step_stats* st = stats ? stats : &local;
+    std::memset(st, 0, sizeof(*st));
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    const double beta = c->opt.newmark_beta, gamma = c->opt.newmark_gamma;
+    // predictor (:845-855): the initial guess is d~
+    launch_newmark_predict(n, dt, beta, gamma, c->vel.as<double>(), c->acc.as<double>(), c->dpred.as<double>(),
+                           c->vpred.as<double>(), s);
+    if (c->has_override) {
+        launch_scale_copy(n, 1.0, c->override_guess.as<double>(), c->d.as<double>(), s);
+        c->has_override = false;
+    } else {
+        launch_scale_copy(n, 1.0, c->dpred.as<double>(), c->d.as<double>(), s);
+    }
+    launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), c->d.as<double>(), s);
+    const double cm = 1.0 / (beta * dt * dt);
+    int rc = newton(c, time, cm, st);
+    if (rc != CMG_SUCCESS) {
+        CMG_HIP_CHECK(hipStreamSynchronize(s));
+        return CMG_FAILED;
+    }
+    // corrector (:984-1015)
+    launch_newmark_correct(n, dt, beta, gamma, c->d.as<double>(), c->dpred.as<double>(), c->vpred.as<double>(),
+                           c->vel.as<double>(), c->acc.as<double>(), s);
+    launch_axpby(n, 1.0, c->d.as<double>(), 1.0, c->nodes.as<double>(), s);
+    CMG_HIP_CHECK(hipStreamSynchronize(s));
+    return CMG_SUCCESS;
+}
+
+}  // extern "C"

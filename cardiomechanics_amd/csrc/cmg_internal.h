@@ -1,0 +1,117 @@
+// Internal declarations of the gfx950 mechanics library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "cmg.h"
+
+namespace cmg {
+
+// Device copy of one material (same meaning as cmg_material).
+struct DevMaterial {
+    int law;
+    int ignore_corrupt;
+    int tension_fn;
+    int pad;
+    double density;
+    double p[12];
+    double tension_max;
+    double fn[12];
+};
+
+// Elements handled by one 64-lane workgroup in the tangent kernel: 9 lanes
+// (one per spatial component pair i, j) per element, 7 elements per wave.
+constexpr int kTanElemsPerWave = 7;
+constexpr int kResidualBlock = 128;
+
+// Per-element data, structure-of-arrays: entry k of element e at [k * E + e].
+struct ElemArgs {
+    int64_t E;
+    const int32_t* conn;     // [npe][E]
+    const double* dNdX;      // [nqp*npe*3][E]  reference dN/dX at the quadrature points
+    const double* bases;     // [nqp*9][E]      fibre frame (columns f, s, n) per quadrature point
+    const double* w;         // [E]  detJ/24 (T10) or volume (T4)
+    const int32_t* matidx;   // [E] or nullptr
+    const double* tact;      // [E]
+    const DevMaterial* mats;
+    const double* x;         // current coordinates [3N]
+    double time;
+    double* fe;              // [E][3npe] element forces
+    void* Ke;                // [E][npe][npe][3][3] element tangents (double or float)
+    int* corrupt;            // smallest element id with det F <= 0 (INT_MAX if none)
+};
+
+struct SurfArgs {
+    int64_t S;
+    int nps;
+    const int32_t* conn;     // [S][nps]
+    const int32_t* group;    // [S]
+    const double* pressure;  // [n_groups]
+    const uint8_t* bcmask;   // [N]
+    const double* x;         // [3N]
+    double* fs;              // [S][3nps]
+    double* Ks;              // [S][3nps][3nps] or nullptr
+    double* vol_partial;     // [S] or nullptr
+};
+
+// ---- launchers (element_kernels.hip) ----
+void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s);
+void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s);
+void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s);
+void launch_t10_shape(int64_t E, const int32_t* conn_aos, const double* X, double* dNdX_soa, double* w, hipStream_t s);
+
+// ---- launchers (assembly_kernels.hip) ----
+// vector gather: f[dof] = sum over incidences; BC components zeroed
+void launch_gather_vector(int64_t N, const int64_t* inc_ptr, const int32_t* inc, const double* fe, int stride_e,
+                          const int64_t* sinc_ptr, const int32_t* sinc, const double* fs, int stride_s,
+                          const uint8_t* bcmask, double* f, hipStream_t s);
+// f += cm * M (d - dt) on free DOFs (Newmark inertia term); M stored as one scalar per block
+void launch_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval, double cm,
+                          const double* d, const double* dpred, const uint8_t* bcmask, double* f, hipStream_t s);
+// BSR values from element/surface contributions + cm*M + identity on constrained DOFs
+void launch_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* contrib, const void* Ke, bool fp32,
+                          const double* Ks, int64_t n_solid_blocks, const int32_t* blkrow, const int32_t* colind,
+                          const double* mval, double cm, const uint8_t* bcmask, void* vals, bool vals_fp32,
+                          hipStream_t s);
+
+// ---- launchers (linalg_kernels.hip) ----
+void launch_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals, bool fp32,
+                 const double* x, double* y, hipStream_t s);
+void launch_spmv_scalar_mass(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval,
+                             const uint8_t* bcmask, const double* x, double* y, hipStream_t s);
+void launch_block_jacobi_setup(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals,
+                               bool fp32, double* dinv, hipStream_t s);
+void launch_block_jacobi_apply(int64_t N, const double* dinv, const double* r, double* z, hipStream_t s);
+// partial dot products of k vectors V[j] (stride ld) with w: out[block][j]; returns #blocks
+int dot_blocks(int64_t n);
+void launch_multidot(int64_t n, int k, const double* V, int64_t ld, const double* w, double* partial, hipStream_t s);
+void launch_reduce_partials(int nblk, int k, const double* partial, double* out, hipStream_t s);
+void launch_multiaxpy(int64_t n, int k, const double* V, int64_t ld, const double* h, double* w, hipStream_t s);
+void launch_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s);      // y = a x + b y
+void launch_waxpby(int64_t n, double a, const double* x, double b, const double* y, double* w, hipStream_t s);
+void launch_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s);
+void launch_fill(int64_t n, double v, double* y, hipStream_t s);
+void launch_fill_int(int64_t n, int v, int* y, hipStream_t s);
+void launch_zero_bc(int64_t N, const uint8_t* bcmask, double* v, hipStream_t s);
+void launch_newmark_predict(int64_t n, double dt, double beta, double gamma, const double* v, const double* a,
+                            double* dpred, double* vpred, hipStream_t s);
+void launch_newmark_correct(int64_t n, double dt, double beta, double gamma, const double* d, const double* dpred,
+                            const double* vpred, double* v, double* a, hipStream_t s);
+
+// error helper
+void set_error(const std::string& msg);
+
+}  // namespace cmg
+
+#define CMG_HIP_CHECK(expr)                                                                 \
+    do {                                                                                    \
+        hipError_t err__ = (expr);                                                          \
+        if (err__ != hipSuccess) {                                                          \
+            cmg::set_error(std::string(#expr) + ": " + hipGetErrorString(err__));           \
+            return CMG_ERR_HIP;                                                             \
+        }                                                                                   \
+    } while (0)

@@ -1,0 +1,723 @@
+// Element kernels for gfx950: T10/T4 residual and residual + analytic tangent.
+//
+// Arithmetic follows the reference element and law routines:
+//   F per quadrature point, local fibre frame   CBElementSolidT10.cpp:266-288
+//   Guccione PK2                                CBConstitutiveModelGuccione.cpp:103-138
+//   Holzapfel PK2 (Jm = pow(J, -2/3) = 1)       CBConstitutiveModelHolzapfel.cpp:118-154
+//   active stress T/sqrt(I4f) f(x)f             CBTensionModel.h:37-50
+//   residual f_a = detJ/24 sum_q dN_a . P_q     CBElementSolidT10.cpp:620-693 (T4: :147-210)
+// The tangent is the exact derivative of that residual (the reference uses central
+// finite differences instead, CBElementSolidT10.cpp:511-567):
+//   K(a,b)_ij = w sum_q sum_{m,P} G_a,m M^{ij}_{mP} G_b,P,  G = B^-1 dN/dX,
+//   M^{ij}_{mP} = (B B^T)_ij S_Pm + sum_{I,K} Ft_iI Ft_jK D_{Im,KP},  Ft = B F_loc,
+// with D = dS/dE of the law in the fibre frame, evaluated in closed form.
+//
+// Lane mapping of the tangent kernel: 64-lane workgroup, 7 elements, 9 lanes per
+// element, lane (i, j) owns K(a,b)_ij for all node pairs (a, b) and computes only
+// the 9 entries M^{ij} it needs, so no LDS and no barriers are used.
+#include <climits>
+
+#include "cmg_internal.h"
+
+namespace cmg {
+namespace {
+
+__device__ __forceinline__ int xcd_swizzle(int bid, int nblk) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
+    // contiguous range of tiles so neighbouring elements share one L2.
+    const int nx = 8;
+    const int full = (nblk / nx) * nx;
+    if (bid >= full) return bid;
+    const int per = nblk / nx;
+    return (bid % nx) * per + bid / nx;
+}
+
+// ---------------------------------------------------------------------------
+// analytic time functions (CBAnalyticFunction.h:53-145), steady Linear window
+// ---------------------------------------------------------------------------
+__device__ double dh_raw(const double* fn, double t) {
+    double c = pow(t / fn[6], fn[8]);
+    c = c / (1.0 + c);
+    double r = pow(t / fn[7], fn[9]);
+    r = 1.0 / (1.0 + r);
+    return c * r;
+}
+
+__device__ double analytic(int type, const double* fn, double time) {
+    const double start = fn[0], stop = fn[1];
+    switch (type) {
+    case CMG_FN_LINEAR:
+        if (time < start) return 0.0;
+        if (time < stop) return (time - start) * 1.0 / (stop - start);
+        return fn[2] != 0.0 ? 1.0 : 0.0;
+    case CMG_FN_SINUS:
+        if ((time >= start) && (time <= stop)) return 0.5 - 0.5 * cos(fn[3] * (time - start) + fn[4]);
+        return 0.0;
+    case CMG_FN_SINUSDRIVER:
+        if ((time >= start) && (time <= stop)) return sin(3.14159 * (time - start) / (stop - start));
+        return 0.0;
+    case CMG_FN_DOUBLEHILL: {
+        if ((time >= start) && (time <= stop)) {
+            double t = time - fn[10];
+            t = (t > 0.0) ? fmod(t, fn[5]) : 0.0;
+            return dh_raw(fn, t) / fn[11];
+        }
+        return 0.0;
+    }
+    default:
+        return 0.0;
+    }
+}
+
+__device__ __forceinline__ double tension_of(const DevMaterial& m, double time, double tact) {
+    if (m.tension_fn == CMG_FN_NONE) return 0.0;
+    return m.tension_max * analytic(m.tension_fn, m.fn, time - tact);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 helpers, row-major, reference term order (Matrix3.h:290-303,482-540)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mul3(const double* a, const double* b, double* r) {
+    r[0] = a[0] * b[0] + a[1] * b[3] + a[2] * b[6];
+    r[1] = a[0] * b[1] + a[1] * b[4] + a[2] * b[7];
+    r[2] = a[0] * b[2] + a[1] * b[5] + a[2] * b[8];
+    r[3] = a[3] * b[0] + a[4] * b[3] + a[5] * b[6];
+    r[4] = a[3] * b[1] + a[4] * b[4] + a[5] * b[7];
+    r[5] = a[3] * b[2] + a[4] * b[5] + a[5] * b[8];
+    r[6] = a[6] * b[0] + a[7] * b[3] + a[8] * b[6];
+    r[7] = a[6] * b[1] + a[7] * b[4] + a[8] * b[7];
+    r[8] = a[6] * b[2] + a[7] * b[5] + a[8] * b[8];
+}
+
+__device__ __forceinline__ void tr3(const double* a, double* r) {
+    r[0] = a[0]; r[1] = a[3]; r[2] = a[6];
+    r[3] = a[1]; r[4] = a[4]; r[5] = a[7];
+    r[6] = a[2]; r[7] = a[5]; r[8] = a[8];
+}
+
+__device__ __forceinline__ double det3(const double* e) {
+    return (e[0] * e[4] * e[8]) + (e[3] * e[7] * e[2]) + (e[6] * e[1] * e[5]) - (e[2] * e[4] * e[6]) -
+           (e[5] * e[7] * e[0]) - (e[8] * e[3] * e[1]);
+}
+
+__device__ __forceinline__ void inv3(const double* e, double* r) {
+    double d = det3(e);
+    if (d == 0.0) {
+        for (int i = 0; i < 9; i++) r[i] = INFINITY;
+        return;
+    }
+    r[0] = (-e[5] * e[7] + e[4] * e[8]) / d;
+    r[1] = (e[2] * e[7] - e[1] * e[8]) / d;
+    r[2] = (-e[2] * e[4] + e[1] * e[5]) / d;
+    r[3] = (e[5] * e[6] - e[3] * e[8]) / d;
+    r[4] = (-e[2] * e[6] + e[0] * e[8]) / d;
+    r[5] = (e[2] * e[3] - e[0] * e[5]) / d;
+    r[6] = (-e[4] * e[6] + e[3] * e[7]) / d;
+    r[7] = (e[1] * e[6] - e[0] * e[7]) / d;
+    r[8] = (-e[1] * e[3] + e[0] * e[4]) / d;
+}
+
+// ---------------------------------------------------------------------------
+// constitutive laws in the fibre frame
+// ---------------------------------------------------------------------------
+// Everything the tangent needs besides S, for
+//   M_mP = BBt S_Pm + czz (Zu)_m (Zw)_P + codot [(u.Zw) Z_mP + (Zu)_P (Zw)_m]
+//        + Guccione: ceq/2 (d_mP h_m + u_P w_m b_Pm) + 2 ceq (Yu)_m (Yw)_P
+//        + Holzapfel: (Xu)_m w_P + mfs a_m b_P
+//        + d00 u_0 w_0 [m=P=0] + d11 u_1 w_1 [m=P=1]
+struct Law {
+    double S[9];
+    double Z[9];     // C^-1
+    double czz, codot;
+    double ceq;      // Guccione C e^Q
+    double Y[9];     // Guccione b (.) E
+    double X[9];     // Holzapfel 2gb(I - I1/3 Z) - 2g/3 Z
+    double d00, d11, mfs;
+};
+
+// returns false for det F <= 0 (CBStatus::CORRUPT_ELEMENT)
+__device__ __forceinline__ bool eval_law(const DevMaterial& m, const double* F, double Ta, bool want_tangent,
+                                         Law& L) {
+    const double J = det3(F);
+    if (!m.ignore_corrupt && (J <= 0.0)) return false;
+    double Ft[9], C[9];
+    tr3(F, Ft);
+    mul3(Ft, F, C);
+    inv3(C, L.Z);
+    L.d00 = 0.0;
+    L.d11 = 0.0;
+    L.mfs = 0.0;
+    if (m.law == CMG_LAW_GUCCIONE) {
+        const double Cg = m.p[0], bf = m.p[1], bt = m.p[2], bfs = m.p[3], k = m.p[4];
+        double e[9];
+        for (int i = 0; i < 9; i++) e[i] = 0.5 * (C[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+        const double Q = bf * e[0] * e[0] + bt * (e[4] * e[4] + e[8] * e[8] + e[5] * e[5] + e[7] * e[7]) +
+                         bfs * (e[1] * e[1] + e[3] * e[3] + e[2] * e[2] + e[6] * e[6]);
+        const double expQ = exp(Q);
+        double* p = L.S;
+        p[0] = Cg * bf * e[0] * expQ;
+        p[1] = Cg * bfs * e[1] * expQ;
+        p[2] = Cg * bfs * e[2] * expQ;
+        p[3] = Cg * bfs * e[3] * expQ;
+        p[4] = Cg * bt * e[4] * expQ;
+        p[5] = Cg * bt * e[5] * expQ;
+        p[6] = Cg * bfs * e[6] * expQ;
+        p[7] = Cg * bt * e[7] * expQ;
+        p[8] = Cg * bt * e[8] * expQ;
+        const double vol = k * (J - 1.0) * J;
+        for (int i = 0; i < 9; i++) p[i] += vol * L.Z[i];
+        if (want_tangent) {
+            const double b[9] = {bf, bfs, bfs, bfs, bt, bt, bfs, bt, bt};
+            L.ceq = Cg * expQ;
+            for (int i = 0; i < 9; i++) L.Y[i] = b[i] * e[i];
+            L.czz = k * (2.0 * J - 1.0) * J;
+            L.codot = -k * (J * J - J);
+        }
+    } else {
+        const double a = m.p[0], b = m.p[1], af = m.p[2], bf = m.p[3], as = m.p[4], bs = m.p[5];
+        const double afs = m.p[6], bfs = m.p[7], kappa = m.p[8], kh = m.p[9];
+        const double I1 = C[0] + C[4] + C[8];
+        const double I4f = C[0], I4s = C[4];
+        const double I8 = 0.5 * (C[1] + C[3]);
+        double Hf, Hs, dHf, dHs;
+        if (kh == 0.0) {
+            Hf = (I4f >= 1.0) ? 1.0 : 0.0;
+            Hs = (I4s >= 1.0) ? 1.0 : 0.0;
+            dHf = 0.0;
+            dHs = 0.0;
+        } else {
+            Hf = 1.0 / (1.0 + exp(-kh * (I4f - 1.0)));
+            Hs = 1.0 / (1.0 + exp(-kh * (I4s - 1.0)));
+            dHf = kh * Hf * (1.0 - Hf);
+            dHs = kh * Hs * (1.0 - Hs);
+        }
+        const double cvol = (kappa / 2.0) * (J - 1.0 / J) * J;
+        const double g = 1.0 * a * exp(b * (1.0 * I1 - 3.0));
+        const double ef = exp(bf * (I4f - 1.0) * (I4f - 1.0));
+        const double es = exp(bs * (I4s - 1.0) * (I4s - 1.0));
+        const double efs = exp(bfs * I8 * I8);
+        const double cf = 2.0 * af * Hf * (I4f - 1.0) * ef;
+        const double cs = 2.0 * as * Hs * (I4s - 1.0) * es;
+        const double cfs = afs * I8 * efs;
+        for (int i = 0; i < 9; i++) {
+            const double id = (i % 4 == 0) ? 1.0 : 0.0;
+            L.S[i] = cvol * L.Z[i] + g * (id - 1.0 / 3.0 * I1 * L.Z[i]);
+        }
+        L.S[0] += cf;
+        L.S[4] += cs;
+        L.S[1] += cfs;
+        L.S[3] += cfs;
+        if (want_tangent) {
+            L.czz = kappa * J * J;
+            L.codot = -(kappa / 2.0) * (J * J - 1.0) + g * I1 / 3.0;
+            for (int i = 0; i < 9; i++) {
+                const double id = (i % 4 == 0) ? 1.0 : 0.0;
+                L.X[i] = 2.0 * g * b * (id - I1 / 3.0 * L.Z[i]) - (2.0 * g / 3.0) * L.Z[i];
+            }
+            L.d00 = 2.0 * 2.0 * af * ef * (dHf * (I4f - 1.0) + Hf * (1.0 + 2.0 * bf * (I4f - 1.0) * (I4f - 1.0)));
+            L.d11 = 2.0 * 2.0 * as * es * (dHs * (I4s - 1.0) + Hs * (1.0 + 2.0 * bs * (I4s - 1.0) * (I4s - 1.0)));
+            L.mfs = afs * efs * (1.0 + 2.0 * bfs * I8 * I8);
+        }
+    }
+    // active stress, FromFunction default coefficients: only T / sqrt(I4f) on ff
+    if (Ta != 0.0) {
+        const double I4f = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+        const double rs = 1. / sqrt(I4f);
+        L.S[0] += Ta * rs;
+        if (want_tangent) L.d00 += -Ta * rs / I4f;
+    }
+    return true;
+}
+
+// M^{ij}_{mP} for the lane's spatial pair (i, j)
+template <typename TT>
+__device__ __forceinline__ void lane_M(const DevMaterial& m, const Law& L, const double* Ftil, const double* B,
+                                       int ii, int jj, TT* M) {
+    const double* u = Ftil + 3 * ii;
+    const double* w = Ftil + 3 * jj;
+    const double bbt = B[3 * ii] * B[3 * jj] + B[3 * ii + 1] * B[3 * jj + 1] + B[3 * ii + 2] * B[3 * jj + 2];
+    double Zu[3], Zw[3];
+    for (int r = 0; r < 3; r++) {
+        Zu[r] = L.Z[3 * r] * u[0] + L.Z[3 * r + 1] * u[1] + L.Z[3 * r + 2] * u[2];
+        Zw[r] = L.Z[3 * r] * w[0] + L.Z[3 * r + 1] * w[1] + L.Z[3 * r + 2] * w[2];
+    }
+    const double uZw = u[0] * Zw[0] + u[1] * Zw[1] + u[2] * Zw[2];
+    double Mm[9];
+    for (int mm = 0; mm < 3; mm++)
+        for (int P = 0; P < 3; P++)
+            Mm[3 * mm + P] = bbt * L.S[3 * P + mm] + L.czz * Zu[mm] * Zw[P] +
+                             L.codot * (uZw * L.Z[3 * mm + P] + Zu[P] * Zw[mm]);
+    if (m.law == CMG_LAW_GUCCIONE) {
+        const double bf = m.p[1], bt = m.p[2], bfs = m.p[3];
+        const double b[9] = {bf, bfs, bfs, bfs, bt, bt, bfs, bt, bt};
+        double Yu[3], Yw[3], h[3];
+        for (int r = 0; r < 3; r++) {
+            Yu[r] = L.Y[3 * r] * u[0] + L.Y[3 * r + 1] * u[1] + L.Y[3 * r + 2] * u[2];
+            Yw[r] = L.Y[3 * r] * w[0] + L.Y[3 * r + 1] * w[1] + L.Y[3 * r + 2] * w[2];
+            h[r] = u[0] * w[0] * b[r] + u[1] * w[1] * b[3 + r] + u[2] * w[2] * b[6 + r];
+        }
+        const double hc = 0.5 * L.ceq, oc = 2.0 * L.ceq;
+        for (int mm = 0; mm < 3; mm++)
+            for (int P = 0; P < 3; P++)
+                Mm[3 * mm + P] += hc * (u[P] * w[mm] * b[3 * P + mm] + ((mm == P) ? h[mm] : 0.0)) +
+                                  oc * Yu[mm] * Yw[P];
+    } else {
+        double Xu[3];
+        for (int r = 0; r < 3; r++) Xu[r] = L.X[3 * r] * u[0] + L.X[3 * r + 1] * u[1] + L.X[3 * r + 2] * u[2];
+        const double av[3] = {u[1], u[0], 0.0}, bv[3] = {w[1], w[0], 0.0};
+        for (int mm = 0; mm < 3; mm++)
+            for (int P = 0; P < 3; P++) Mm[3 * mm + P] += Xu[mm] * w[P] + L.mfs * av[mm] * bv[P];
+        Mm[4] += L.d11 * u[1] * w[1];
+    }
+    Mm[0] += L.d00 * u[0] * w[0];
+    for (int r = 0; r < 9; r++) M[r] = (TT)Mm[r];
+}
+
+template <int NPE>
+__device__ __forceinline__ void load_g(const ElemArgs& a, int64_t e, int q, double (*g)[3]) {
+    const int64_t E = a.E;
+#pragma unroll
+    for (int k = 0; k < NPE; k++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) g[k][d] = a.dNdX[((int64_t)(q * NPE + k) * 3 + d) * E + e];
+}
+
+template <int NPE>
+__device__ __forceinline__ void local_F(const double* x, const double (*g)[3], const double* B, double* Binv,
+                                        double* Floc) {
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < NPE; k++) s += g[k][j] * x[3 * k + i];
+            F[3 * i + j] = s;
+        }
+    double Bt[9], BinvT[9], T1[9];
+    inv3(B, Binv);
+    tr3(B, Bt);
+    tr3(Binv, BinvT);
+    mul3(Bt, F, T1);
+    mul3(T1, BinvT, Floc);
+}
+
+// ---------------------------------------------------------------------------
+// residual only: one thread per element, FP64
+// ---------------------------------------------------------------------------
+template <int NPE, int NQP>
+__global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a) {
+    const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int64_t e = (int64_t)bid * kResidualBlock + threadIdx.x;
+    if (e >= a.E) return;
+    const int64_t E = a.E;
+    const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
+    const double Ta = tension_of(m, a.time, a.tact[e]);
+    double x[3 * NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; k++) {
+        const int64_t n = a.conn[(int64_t)k * E + e];
+        x[3 * k] = a.x[3 * n];
+        x[3 * k + 1] = a.x[3 * n + 1];
+        x[3 * k + 2] = a.x[3 * n + 2];
+    }
+    double t[NPE][3];
+#pragma unroll
+    for (int k = 0; k < NPE; k++) t[k][0] = t[k][1] = t[k][2] = 0.0;
+    for (int q = 0; q < NQP; q++) {
+        double g[NPE][3];
+        load_g<NPE>(a, e, q, g);
+        double B[9], Binv[9], Floc[9];
+#pragma unroll
+        for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
+        local_F<NPE>(x, g, B, Binv, Floc);
+        Law L;
+        if (!eval_law(m, Floc, Ta, false, L)) {
+            atomicMin(a.corrupt, (int)e);
+            return;
+        }
+        // stress = (F S)^T (T10, :649) or S F^T (T4, :176); S is symmetric so both agree
+        double FS[9], P1[9], P[9], BinvT[9], Bt[9], St[9];
+        mul3(Floc, L.S, FS);
+        tr3(FS, St);
+        tr3(Binv, BinvT);
+        tr3(B, Bt);
+        mul3(BinvT, St, P1);
+        mul3(P1, Bt, P);
+#pragma unroll
+        for (int k = 0; k < NPE; k++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) t[k][c] += g[k][0] * P[0 + c] + g[k][1] * P[3 + c] + g[k][2] * P[6 + c];
+    }
+    const double V = a.w[e];
+    double* fe = a.fe + e * (3 * NPE);
+#pragma unroll
+    for (int k = 0; k < NPE; k++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) fe[3 * k + c] = V * t[k][c];
+}
+
+// ---------------------------------------------------------------------------
+// residual + tangent: 9 lanes per element (see file header)
+// ---------------------------------------------------------------------------
+template <int NPE, int NQP, typename TT>
+__global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a) {
+    const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int slot = lane / 9;
+    const int ij = lane - 9 * slot;
+    const int ii = ij / 3, jj = ij - 3 * (ij / 3);
+    const int64_t e = (int64_t)bid * kTanElemsPerWave + slot;
+    if (slot >= kTanElemsPerWave || e >= a.E) return;
+    const int64_t E = a.E;
+    const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
+    const double Ta = tension_of(m, a.time, a.tact[e]);
+    double x[3 * NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; k++) {
+        const int64_t n = a.conn[(int64_t)k * E + e];
+        x[3 * k] = a.x[3 * n];
+        x[3 * k + 1] = a.x[3 * n + 1];
+        x[3 * k + 2] = a.x[3 * n + 2];
+    }
+    TT acc[NPE * NPE];
+#pragma unroll
+    for (int r = 0; r < NPE * NPE; r++) acc[r] = (TT)0;
+    double fr[NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; k++) fr[k] = 0.0;
+    for (int q = 0; q < NQP; q++) {
+        double B[9], Binv[9], Floc[9];
+        TT G[NPE][3];
+        Law L;
+        {
+            double g[NPE][3];
+            load_g<NPE>(a, e, q, g);
+#pragma unroll
+            for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
+            local_F<NPE>(x, g, B, Binv, Floc);
+            if (!eval_law(m, Floc, Ta, true, L)) {
+                atomicMin(a.corrupt, (int)e);
+                return;
+            }
+            // G_k = B^-1 dN_k/dX
+#pragma unroll
+            for (int k = 0; k < NPE; k++)
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    G[k][r] = (TT)(Binv[3 * r] * g[k][0] + Binv[3 * r + 1] * g[k][1] + Binv[3 * r + 2] * g[k][2]);
+            if (jj == 0) {
+                // residual component ii in the reference form (FP64, exact dN/dX)
+                double FS[9], St[9], BinvT[9], Bt[9], P1[9], P[9];
+                mul3(Floc, L.S, FS);
+                tr3(FS, St);
+                tr3(Binv, BinvT);
+                tr3(B, Bt);
+                mul3(BinvT, St, P1);
+                mul3(P1, Bt, P);
+#pragma unroll
+                for (int k = 0; k < NPE; k++) fr[k] += g[k][0] * P[ii] + g[k][1] * P[3 + ii] + g[k][2] * P[6 + ii];
+            }
+        }
+        double Ftil[9];
+        mul3(B, Floc, Ftil);
+        TT M[9];
+        lane_M<TT>(m, L, Ftil, B, ii, jj, M);
+#pragma unroll
+        for (int b = 0; b < NPE; b++) {
+            const TT mg0 = M[0] * G[b][0] + M[1] * G[b][1] + M[2] * G[b][2];
+            const TT mg1 = M[3] * G[b][0] + M[4] * G[b][1] + M[5] * G[b][2];
+            const TT mg2 = M[6] * G[b][0] + M[7] * G[b][1] + M[8] * G[b][2];
+#pragma unroll
+            for (int aa = 0; aa < NPE; aa++) acc[aa * NPE + b] += G[aa][0] * mg0 + G[aa][1] * mg1 + G[aa][2] * mg2;
+        }
+    }
+    const double V = a.w[e];
+    TT* Ke = static_cast<TT*>(a.Ke) + e * (9 * NPE * NPE);
+    const TT Vt = (TT)V;
+#pragma unroll
+    for (int r = 0; r < NPE * NPE; r++) Ke[9 * r + ij] = Vt * acc[r];
+    if (jj == 0) {
+        double* fe = a.fe + e * (3 * NPE);
+#pragma unroll
+        for (int k = 0; k < NPE; k++) fe[3 * k + ii] = V * fr[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// surface pressure loads (CBElementSurfaceT3.cpp:257-315, CBElementSurfaceT6.cpp:248-303)
+// with exact tangents by forward-mode dual numbers, and the volume contribution
+// (CBElementSurfaceT3.cpp:151-191, CBElementSurfaceT6.cpp:191-241)
+// ---------------------------------------------------------------------------
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator*(double s, Dual b) { return {s * b.v, s * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+    return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)};
+}
+__device__ __forceinline__ Dual dsqrt(Dual a) {
+    double s = sqrt(a.v);
+    return {s, a.d / (2.0 * s)};
+}
+__device__ __forceinline__ double val(double a) { return a; }
+__device__ __forceinline__ double val(Dual a) { return a.v; }
+__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
+
+template <typename T>
+__device__ __forceinline__ void cross3(const T* a, const T* b, T* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <typename T>
+__device__ void t3_forces(const T* x, double p, T* f) {
+    T ab[3], ac[3], n[3];
+    for (int i = 0; i < 3; i++) {
+        ab[i] = x[i] - x[3 + i];
+        ac[i] = x[i] - x[6 + i];
+    }
+    cross3(ab, ac, n);
+    T nn = dsqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    T area = 0.5 * nn;
+    if (val(nn) != 0.0)
+        for (int i = 0; i < 3; i++) n[i] = n[i] / nn;
+    T s = (-p * (1. / 3.)) * area;
+    for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++) f[3 * a + c] = s * n[c];
+}
+
+template <typename T>
+__device__ void tri_normal(const T* v0, const T* v1, const T* v2, T* n, T& area) {
+    T a[3], b[3];
+    for (int i = 0; i < 3; i++) {
+        a[i] = v1[i] - v0[i];
+        b[i] = v2[i] - v0[i];
+    }
+    cross3(a, b, n);
+    T nn = dsqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    area = 0.5 * nn;
+    if (val(nn) != 0.0)
+        for (int i = 0; i < 3; i++) n[i] = n[i] / nn;
+}
+
+template <typename T>
+__device__ void t6_forces(const T* x, double p, T* f) {
+    T t1[3], t2[3], t3[3], t4[3], A1, A2, A3, A4;
+    tri_normal(x + 0, x + 9, x + 15, t1, A1);
+    tri_normal(x + 9, x + 12, x + 15, t2, A2);
+    tri_normal(x + 9, x + 3, x + 12, t3, A3);
+    tri_normal(x + 15, x + 12, x + 6, t4, A4);
+    T area = A1 + A2 + A3 + A4;
+    T nv[3][3];
+    for (int i = 0; i < 3; i++) {
+        nv[0][i] = t1[i] + t2[i];
+        nv[1][i] = t3[i] + t2[i];
+        nv[2][i] = t4[i] + t2[i];
+    }
+    for (int k = 0; k < 3; k++) {
+        T nn = dsqrt(nv[k][0] * nv[k][0] + nv[k][1] * nv[k][1] + nv[k][2] * nv[k][2]);
+        if (val(nn) != 0.0)
+            for (int i = 0; i < 3; i++) nv[k][i] = nv[k][i] / nn;
+    }
+    const double l[3][3] = {{2. / 3., 1. / 6., 1. / 6.}, {1. / 6., 2. / 3., 1. / 6.}, {1. / 6., 1. / 6., 2. / 3.}};
+    T s = (-p / 3.0) * area;
+    for (int i = 0; i < 6; i++) {
+        double N[3];
+        for (int k = 0; k < 3; k++) {
+            const double l1 = l[k][0], l2 = l[k][1], l3 = l[k][2];
+            N[k] = (i == 0) ? l1 * (2 * l1 - 1)
+                 : (i == 1) ? l2 * (2 * l2 - 1)
+                 : (i == 2) ? l3 * (2 * l3 - 1)
+                 : (i == 3) ? 4 * l1 * l2
+                 : (i == 4) ? 4 * l1 * l3
+                            : 4 * l2 * l3;
+        }
+        for (int c = 0; c < 3; c++) f[3 * i + c] = s * (N[0] * nv[0][c] + N[1] * nv[1][c] + N[2] * nv[2][c]);
+    }
+}
+
+template <int NPS>
+__global__ __launch_bounds__(64) void k_surface(const SurfArgs a, bool tangent) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.S) return;
+    const int32_t* cn = a.conn + s * NPS;
+    const double p = a.pressure[a.group[s]];
+    double x[3 * NPS];
+    uint8_t bc[3 * NPS];
+    for (int k = 0; k < NPS; k++) {
+        const int64_t n = cn[k];
+        const uint8_t mask = a.bcmask ? a.bcmask[n] : 0;
+        for (int c = 0; c < 3; c++) {
+            x[3 * k + c] = a.x[3 * n + c];
+            bc[3 * k + c] = (mask >> c) & 1;
+        }
+    }
+    double f[3 * NPS];
+    if (NPS == 3)
+        t3_forces<double>(x, p, f);
+    else
+        t6_forces<double>(x, p, f);
+    for (int r = 0; r < 3 * NPS; r++) a.fs[s * (3 * NPS) + r] = bc[r] ? 0.0 : f[r];
+    if (a.vol_partial) {
+        double v;
+        if (NPS == 3) {
+            double aa[3] = {x[3] - x[0], x[4] - x[1], x[5] - x[2]};
+            double bb[3] = {x[6] - x[0], x[7] - x[1], x[8] - x[2]};
+            double c[3];
+            cross3(aa, bb, c);
+            v = (x[0] * c[0] + x[1] * c[1] + x[2] * c[2]) / 6.0;
+        } else {
+            double a1[3], a2[3], a3[3], a4[3], a5[3];
+            for (int i = 0; i < 3; i++) {
+                a1[i] = x[9 + i] - x[i];
+                a2[i] = x[3 + i] - x[i];
+                a3[i] = x[12 + i] - x[i];
+                a4[i] = x[6 + i] - x[i];
+                a5[i] = x[15 + i] - x[i];
+            }
+            double c1[3], c2[3], c3[3], c4[3];
+            cross3(a1, a2, c1);
+            cross3(a2, a3, c2);
+            cross3(a3, a4, c3);
+            cross3(a4, a5, c4);
+            double v1 = x[0] * c1[0] + x[1] * c1[1] + x[2] * c1[2];
+            double v2 = x[0] * c2[0] + x[1] * c2[1] + x[2] * c2[2];
+            double v3 = x[0] * c3[0] + x[1] * c3[1] + x[2] * c3[2];
+            double v4 = x[0] * c4[0] + x[1] * c4[1] + x[2] * c4[2];
+            v = (v1 + v2 + v3 + v4) / 6.0;
+        }
+        a.vol_partial[s] = v;
+    }
+    if (tangent && a.Ks) {
+        constexpr int n = 3 * NPS;
+        double* K = a.Ks + s * (n * n);
+        for (int c = 0; c < n; c++) {
+            Dual xd[n], fd[n];
+            for (int i = 0; i < n; i++) xd[i] = {x[i], (i == c) ? 1.0 : 0.0};
+            if (NPS == 3)
+                t3_forces<Dual>(xd, p, fd);
+            else
+                t6_forces<Dual>(xd, p, fd);
+            for (int r = 0; r < n; r++) K[n * r + c] = (bc[r] || bc[c]) ? 0.0 : fd[r].d;
+        }
+    }
+}
+
+// dN/dX at the 4 quadrature points (SoA) and detJ/24 per element,
+// CBElementSolidT10::CalcShapeFunctionDerivatives (:94-264)
+__device__ double t10_shape_point(const double* X, double l1, double l2, double l3, double l4, double* out) {
+    double Jx[4], Jy[4], Jz[4];
+    const double* c[10];
+    for (int k = 0; k < 10; k++) c[k] = X + 3 * k;
+    for (int d = 0; d < 3; d++) {
+        double* J = (d == 0) ? Jx : (d == 1) ? Jy : Jz;
+        J[0] = 4.0 * (c[0][d] * (l1 - 0.25) + c[4][d] * l2 + c[6][d] * l3 + c[7][d] * l4);
+        J[1] = 4.0 * (c[4][d] * l1 + c[1][d] * (l2 - 0.25) + c[5][d] * l3 + c[8][d] * l4);
+        J[2] = 4.0 * (c[6][d] * l1 + c[5][d] * l2 + c[2][d] * (l3 - 0.25) + c[9][d] * l4);
+        J[3] = 4.0 * (c[7][d] * l1 + c[8][d] * l2 + c[9][d] * l3 + c[3][d] * (l4 - 0.25));
+    }
+#define D_(v, i, j) (v[(i)-1] - v[(j)-1])
+    const double detJ = D_(Jx, 2, 1) * (D_(Jy, 2, 3) * D_(Jz, 3, 4) - D_(Jy, 3, 4) * D_(Jz, 2, 3)) +
+                        D_(Jx, 3, 2) * (D_(Jy, 3, 4) * D_(Jz, 1, 2) - D_(Jy, 1, 2) * D_(Jz, 3, 4)) +
+                        D_(Jx, 4, 3) * (D_(Jy, 1, 2) * D_(Jz, 2, 3) - D_(Jy, 2, 3) * D_(Jz, 1, 2));
+    const double av[4] = {D_(Jy, 4, 2) * D_(Jz, 3, 2) - D_(Jy, 3, 2) * D_(Jz, 4, 2),
+                          D_(Jy, 3, 1) * D_(Jz, 4, 3) - D_(Jy, 3, 4) * D_(Jz, 1, 3),
+                          D_(Jy, 2, 4) * D_(Jz, 1, 4) - D_(Jy, 1, 4) * D_(Jz, 2, 4),
+                          D_(Jy, 1, 3) * D_(Jz, 2, 1) - D_(Jy, 1, 2) * D_(Jz, 3, 1)};
+    const double bv[4] = {D_(Jx, 3, 2) * D_(Jz, 4, 2) - D_(Jx, 4, 2) * D_(Jz, 3, 2),
+                          D_(Jx, 4, 3) * D_(Jz, 3, 1) - D_(Jx, 1, 3) * D_(Jz, 3, 4),
+                          D_(Jx, 1, 4) * D_(Jz, 2, 4) - D_(Jx, 2, 4) * D_(Jz, 1, 4),
+                          D_(Jx, 2, 1) * D_(Jz, 1, 3) - D_(Jx, 3, 1) * D_(Jz, 1, 2)};
+    const double cv[4] = {D_(Jx, 4, 2) * D_(Jy, 3, 2) - D_(Jx, 3, 2) * D_(Jy, 4, 2),
+                          D_(Jx, 3, 1) * D_(Jy, 4, 3) - D_(Jx, 3, 4) * D_(Jy, 1, 3),
+                          D_(Jx, 2, 4) * D_(Jy, 1, 4) - D_(Jx, 1, 4) * D_(Jy, 2, 4),
+                          D_(Jx, 1, 3) * D_(Jy, 2, 1) - D_(Jx, 1, 2) * D_(Jy, 3, 1)};
+#undef D_
+    const double fac = 4.0 / detJ;
+    const double* comp[3] = {av, bv, cv};
+    for (int d = 0; d < 3; d++) {
+        const double* v = comp[d];
+        out[0 + d] = fac * (l1 - 0.25) * v[0];
+        out[3 + d] = fac * (l2 - 0.25) * v[1];
+        out[6 + d] = fac * (l3 - 0.25) * v[2];
+        out[9 + d] = fac * (l4 - 0.25) * v[3];
+        out[12 + d] = fac * (l1 * v[1] + l2 * v[0]);
+        out[15 + d] = fac * (l2 * v[2] + l3 * v[1]);
+        out[18 + d] = fac * (l3 * v[0] + l1 * v[2]);
+        out[21 + d] = fac * (l1 * v[3] + l4 * v[0]);
+        out[24 + d] = fac * (l2 * v[3] + l4 * v[1]);
+        out[27 + d] = fac * (l3 * v[3] + l4 * v[2]);
+    }
+    return detJ;
+}
+
+__global__ void k_t10_shape(int64_t E, const int32_t* conn, const double* Xg, double* dNdX, double* w) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double X[30], out[30];
+    for (int k = 0; k < 10; k++) {
+        const int64_t n = conn[e * 10 + k];
+        for (int c = 0; c < 3; c++) X[3 * k + c] = Xg[3 * n + c];
+    }
+    const double qa = (5 + 3 * sqrt(5.0)) / 20, qb = (5 - sqrt(5.0)) / 20;
+    const double L[5][4] = {{qa, qb, qb, qb}, {qb, qa, qb, qb}, {qb, qb, qa, qb}, {qb, qb, qb, qa},
+                            {0.25, 0.25, 0.25, 0.25}};
+    double detJ = 0.0;
+    for (int q = 0; q < 5; q++) {
+        detJ = t10_shape_point(X, L[q][0], L[q][1], L[q][2], L[q][3], out);
+        if (q < 4)
+            for (int r = 0; r < 30; r++) dNdX[(int64_t)(q * 30 + r) * E + e] = out[r];
+    }
+    w[e] = detJ / 24.0;
+}
+
+}  // namespace
+
+void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s) {
+    if (a.E == 0) return;
+    const int nblk = (int)((a.E + kResidualBlock - 1) / kResidualBlock);
+    if (npe == 10)
+        hipLaunchKernelGGL((k_residual<10, 4>), dim3(nblk), dim3(kResidualBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_residual<4, 1>), dim3(nblk), dim3(kResidualBlock), 0, s, a);
+}
+
+void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s) {
+    if (a.E == 0) return;
+    const int nblk = (int)((a.E + kTanElemsPerWave - 1) / kTanElemsPerWave);
+    if (npe == 10) {
+        if (fp32)
+            hipLaunchKernelGGL((k_residual_tangent<10, 4, float>), dim3(nblk), dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_residual_tangent<10, 4, double>), dim3(nblk), dim3(64), 0, s, a);
+    } else {
+        if (fp32)
+            hipLaunchKernelGGL((k_residual_tangent<4, 1, float>), dim3(nblk), dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_residual_tangent<4, 1, double>), dim3(nblk), dim3(64), 0, s, a);
+    }
+}
+
+void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s) {
+    if (a.S == 0) return;
+    const int nblk = (int)((a.S + 63) / 64);
+    if (a.nps == 3)
+        hipLaunchKernelGGL((k_surface<3>), dim3(nblk), dim3(64), 0, s, a, tangent);
+    else
+        hipLaunchKernelGGL((k_surface<6>), dim3(nblk), dim3(64), 0, s, a, tangent);
+}
+
+void launch_t10_shape(int64_t E, const int32_t* conn_aos, const double* X, double* dNdX_soa, double* w,
+                      hipStream_t s) {
+    if (E == 0) return;
+    const int nblk = (int)((E + 127) / 128);
+    hipLaunchKernelGGL(k_t10_shape, dim3(nblk), dim3(128), 0, s, E, conn_aos, X, dNdX_soa, w);
+}
+
+}  // namespace cmg

@@ -1,0 +1,315 @@
+// Sparse and vector kernels of the Krylov solve (gfx950): 3x3 block-CSR SpMV,
+// nodal block Jacobi, fused multi-dot / multi-axpy for Gram-Schmidt, BLAS-1, and
+// the Newmark predictor/corrector (CBSolverNewmarkBeta.cpp:845-852, 984-1015).
+// All reductions are two-stage with a fixed order (no atomics): bitwise
+// reproducible from run to run.
+#include "cmg_internal.h"
+
+namespace cmg {
+namespace {
+
+constexpr int kSpmvGroup = 8;   // lanes per block row
+
+template <typename VT>
+__global__ __launch_bounds__(256) void k_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind,
+                                              const VT* vals, const double* x, double* y) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = gt / kSpmvGroup;
+    const int sub = (int)(gt % kSpmvGroup);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (r < N) {
+        const int64_t end = rowptr[r + 1];
+        for (int64_t k = rowptr[r] + sub; k < end; k += kSpmvGroup) {
+            const int64_t c = colind[k];
+            const VT* a = vals + 9 * k;
+            const double x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
+            s0 += (double)a[0] * x0 + (double)a[1] * x1 + (double)a[2] * x2;
+            s1 += (double)a[3] * x0 + (double)a[4] * x1 + (double)a[5] * x2;
+            s2 += (double)a[6] * x0 + (double)a[7] * x1 + (double)a[8] * x2;
+        }
+    }
+#pragma unroll
+    for (int o = kSpmvGroup / 2; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, kSpmvGroup);
+        s1 += __shfl_xor(s1, o, kSpmvGroup);
+        s2 += __shfl_xor(s2, o, kSpmvGroup);
+    }
+    if (r < N && sub == 0) {
+        y[3 * r] = s0;
+        y[3 * r + 1] = s1;
+        y[3 * r + 2] = s2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_spmv_scalar_mass(int64_t N, const int64_t* rowptr, const int32_t* colind,
+                                                          const double* mval, const uint8_t* bcmask,
+                                                          const double* x, double* y) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
+        const int64_t c = colind[k];
+        s0 += mval[k] * x[3 * c];
+        s1 += mval[k] * x[3 * c + 1];
+        s2 += mval[k] * x[3 * c + 2];
+    }
+    const uint8_t m = bcmask ? bcmask[r] : 0;
+    y[3 * r] = (m & 1) ? 0.0 : s0;
+    y[3 * r + 1] = (m & 2) ? 0.0 : s1;
+    y[3 * r + 2] = (m & 4) ? 0.0 : s2;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(256) void k_bj_setup(int64_t N, const int64_t* rowptr, const int32_t* colind,
+                                                  const VT* vals, double* dinv) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    double a[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
+        if (colind[k] == r) {
+            for (int q = 0; q < 9; q++) a[q] = (double)vals[9 * k + q];
+            break;
+        }
+    }
+    const double d = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+    const double id = (d != 0.0) ? 1.0 / d : 0.0;
+    double* o = dinv + 9 * r;
+    o[0] = (a[4] * a[8] - a[5] * a[7]) * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = (a[5] * a[6] - a[3] * a[8]) * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = (a[3] * a[7] - a[4] * a[6]) * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+__global__ __launch_bounds__(256) void k_bj_apply(int64_t N, const double* dinv, const double* r, double* z) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const double* a = dinv + 9 * n;
+    const double r0 = r[3 * n], r1 = r[3 * n + 1], r2 = r[3 * n + 2];
+    z[3 * n] = a[0] * r0 + a[1] * r1 + a[2] * r2;
+    z[3 * n + 1] = a[3] * r0 + a[4] * r1 + a[5] * r2;
+    z[3 * n + 2] = a[6] * r0 + a[7] * r1 + a[8] * r2;
+}
+
+// Multi-dot: partial[b][j] = sum over the b-th chunk of V_j . w, j < k.
+constexpr int kDotThreads = 256;
+constexpr int kDotPerThread = 8;
+constexpr int kDotChunk = kDotThreads * kDotPerThread;
+constexpr int kDotGroup = 8;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kDotThreads / 64; i++) s += sh[i];
+    __syncthreads();
+    return s;
+}
+
+__global__ __launch_bounds__(kDotThreads) void k_multidot(int64_t n, int k, const double* V, int64_t ld,
+                                                          const double* w, double* partial) {
+    __shared__ double sh[kDotThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kDotChunk;
+    double wv[kDotPerThread];
+#pragma unroll
+    for (int q = 0; q < kDotPerThread; q++) {
+        const int64_t i = base + (int64_t)q * kDotThreads + threadIdx.x;
+        wv[q] = (i < n) ? w[i] : 0.0;
+    }
+    for (int j0 = 0; j0 < k; j0 += kDotGroup) {
+        double acc[kDotGroup];
+#pragma unroll
+        for (int g = 0; g < kDotGroup; g++) acc[g] = 0.0;
+#pragma unroll
+        for (int g = 0; g < kDotGroup; g++) {
+            if (j0 + g < k) {
+                const double* v = V + (int64_t)(j0 + g) * ld;
+#pragma unroll
+                for (int q = 0; q < kDotPerThread; q++) {
+                    const int64_t i = base + (int64_t)q * kDotThreads + threadIdx.x;
+                    if (i < n) acc[g] += v[i] * wv[q];
+                }
+            }
+        }
+        for (int g = 0; g < kDotGroup && j0 + g < k; g++) {
+            const double s = block_sum(acc[g], sh);
+            if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * k + j0 + g] = s;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_partials(int nblk, int k, const double* partial, double* out) {
+    __shared__ double sh[4];
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(int64_t)b * k + j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[j] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void k_multiaxpy(int64_t n, int k, const double* V, int64_t ld, const double* h,
+                                                   double* w) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = w[i];
+    for (int j = 0; j < k; j++) s -= h[j] * V[(int64_t)j * ld + i];
+    w[i] = s;
+}
+
+__global__ __launch_bounds__(256) void k_axpby(int64_t n, double a, const double* x, double b, double* y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+__global__ __launch_bounds__(256) void k_waxpby(int64_t n, double a, const double* x, double b, const double* y,
+                                                double* w) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = a * x[i] + b * y[i];
+}
+
+__global__ __launch_bounds__(256) void k_scale_copy(int64_t n, double a, const double* x, double* y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i];
+}
+
+__global__ __launch_bounds__(256) void k_fill(int64_t n, double v, double* y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+
+__global__ __launch_bounds__(64) void k_fill_int(int64_t n, int v, int* y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+
+// CBSolverNewmarkBeta.cpp:845-852: d~ = dt v + dt^2/2 (1-2 beta) a ; v~ = v + (1-gamma) dt a
+__global__ __launch_bounds__(256) void k_newmark_predict(int64_t n, double dt, double beta, double gamma,
+                                                         const double* v, const double* a, double* dp, double* vp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dp[i] = dt * v[i] + dt * dt * 0.5 * (1 - 2 * beta) * a[i];
+    vp[i] = 1.0 * v[i] + (1 - gamma) * dt * a[i];
+}
+
+// CBSolverNewmarkBeta.cpp:984-991: a = (d - d~)/(beta dt^2) ; v = v~ + gamma dt a
+__global__ __launch_bounds__(256) void k_newmark_correct(int64_t n, double dt, double beta, double gamma,
+                                                         const double* d, const double* dp, const double* vp,
+                                                         double* v, double* a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ai = 1.0 / (beta * dt * dt) * d[i] + -1.0 / (beta * dt * dt) * dp[i];
+    a[i] = ai;
+    v[i] = vp[i] + gamma * dt * ai;
+}
+
+__global__ __launch_bounds__(256) void k_zero_bc(int64_t N, const uint8_t* bcmask, double* v) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const uint8_t m = bcmask[r];
+    if (m & 1) v[3 * r] = 0.0;
+    if (m & 2) v[3 * r + 1] = 0.0;
+    if (m & 4) v[3 * r + 2] = 0.0;
+}
+
+inline dim3 grid1(int64_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+
+}  // namespace
+
+void launch_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals, bool fp32,
+                 const double* x, double* y, hipStream_t s) {
+    if (N == 0) return;
+    const dim3 g = grid1(N * kSpmvGroup);
+    if (fp32)
+        hipLaunchKernelGGL((k_spmv<float>), g, dim3(256), 0, s, N, rowptr, colind, (const float*)vals, x, y);
+    else
+        hipLaunchKernelGGL((k_spmv<double>), g, dim3(256), 0, s, N, rowptr, colind, (const double*)vals, x, y);
+}
+
+void launch_spmv_scalar_mass(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval,
+                             const uint8_t* bcmask, const double* x, double* y, hipStream_t s) {
+    if (N == 0) return;
+    hipLaunchKernelGGL(k_spmv_scalar_mass, grid1(N), dim3(256), 0, s, N, rowptr, colind, mval, bcmask, x, y);
+}
+
+void launch_block_jacobi_setup(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals, bool fp32,
+                               double* dinv, hipStream_t s) {
+    if (N == 0) return;
+    if (fp32)
+        hipLaunchKernelGGL((k_bj_setup<float>), grid1(N), dim3(256), 0, s, N, rowptr, colind, (const float*)vals,
+                           dinv);
+    else
+        hipLaunchKernelGGL((k_bj_setup<double>), grid1(N), dim3(256), 0, s, N, rowptr, colind, (const double*)vals,
+                           dinv);
+}
+
+void launch_block_jacobi_apply(int64_t N, const double* dinv, const double* r, double* z, hipStream_t s) {
+    if (N == 0) return;
+    hipLaunchKernelGGL(k_bj_apply, grid1(N), dim3(256), 0, s, N, dinv, r, z);
+}
+
+int dot_blocks(int64_t n) { return (int)((n + kDotChunk - 1) / kDotChunk); }
+
+void launch_multidot(int64_t n, int k, const double* V, int64_t ld, const double* w, double* partial, hipStream_t s) {
+    const int nb = dot_blocks(n);
+    if (nb == 0 || k == 0) return;
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kDotThreads), 0, s, n, k, V, ld, w, partial);
+}
+
+void launch_reduce_partials(int nblk, int k, const double* partial, double* out, hipStream_t s) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(256), 0, s, nblk, k, partial, out);
+}
+
+void launch_multiaxpy(int64_t n, int k, const double* V, int64_t ld, const double* h, double* w, hipStream_t s) {
+    if (n == 0 || k == 0) return;
+    hipLaunchKernelGGL(k_multiaxpy, grid1(n), dim3(256), 0, s, n, k, V, ld, h, w);
+}
+
+void launch_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, s, n, a, x, b, y);
+}
+
+void launch_waxpby(int64_t n, double a, const double* x, double b, const double* y, double* w, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_waxpby, grid1(n), dim3(256), 0, s, n, a, x, b, y, w);
+}
+
+void launch_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_scale_copy, grid1(n), dim3(256), 0, s, n, a, x, y);
+}
+
+void launch_fill(int64_t n, double v, double* y, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill, grid1(n), dim3(256), 0, s, n, v, y);
+}
+
+void launch_fill_int(int64_t n, int v, int* y, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_int, grid1(n, 64), dim3(64), 0, s, n, v, y);
+}
+
+void launch_zero_bc(int64_t N, const uint8_t* bcmask, double* v, hipStream_t s) {
+    if (N) hipLaunchKernelGGL(k_zero_bc, grid1(N), dim3(256), 0, s, N, bcmask, v);
+}
+
+void launch_newmark_predict(int64_t n, double dt, double beta, double gamma, const double* v, const double* a,
+                            double* dpred, double* vpred, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_newmark_predict, grid1(n), dim3(256), 0, s, n, dt, beta, gamma, v, a, dpred, vpred);
+}
+
+void launch_newmark_correct(int64_t n, double dt, double beta, double gamma, const double* d, const double* dpred,
+                            const double* vpred, double* v, double* a, hipStream_t s) {
+    if (n)
+        hipLaunchKernelGGL(k_newmark_correct, grid1(n), dim3(256), 0, s, n, dt, beta, gamma, d, dpred, vpred, v, a);
+}
+
+}  // namespace cmg

@@ -1,0 +1,155 @@
+"""Material, tension and time-function parameters (host side).
+
+Mirrors how the reference reads them (key names kept so settings round-trip):
+  Guccione   CBConstitutiveModelGuccione::Init   (CBConstitutiveModelGuccione.cpp:20-71)
+  Holzapfel  CBConstitutiveModelHolzapfel::Init  (CBConstitutiveModelHolzapfel.cpp:18-85)
+  density    CBMaterial::Init                    (CBMaterial.cpp:19-74)
+  tension    CBTensionFromFunction               (CBTensionFromFunction.cpp:36-56)
+  functions  CBAnalyticFunction subclasses       (CBData/CBAnalyticFunction.h:39-176)
+Per-material keys fall back to Materials.Mat_Default.<key> when absent.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+LAW_GUCCIONE = 0
+LAW_HOLZAPFEL = 1
+FN_NONE, FN_LINEAR, FN_SINUS, FN_SINUSDRIVER, FN_DOUBLEHILL = 0, 1, 2, 3, 4
+_FN_NAMES = {"None": FN_NONE, "Linear": FN_LINEAR, "Sinus": FN_SINUS, "SinusDriver": FN_SINUSDRIVER,
+             "DoubleHill": FN_DOUBLEHILL}
+
+
+@dataclass
+class Material:
+    law: int = LAW_GUCCIONE
+    # Guccione: C, bf, bt, bfs, K            Holzapfel: a, b, af, bf, as, bs, afs, bfs, kappa, k
+    params: list = field(default_factory=list)
+    density: float = 0.0
+    ignore_corrupt: int = 0
+    tension_fn: int = FN_NONE
+    tension_max: float = 0.0
+    # startTime, stopTime, keepMaxValue, omega, phi, DoubleHill period, contrTimeOffset,
+    # relaxTimeOffset, contrRateConst, relaxRateConst, onsetTime, driverMax
+    fn: list = field(default_factory=lambda: [0.0] * 12)
+
+
+def guccione(C, bf, bt, bfs, K, density=1082.0, a_scale=1.0, b_scale=1.0, **kw) -> Material:
+    """Guccione law with the aScale/bScale hooks applied as in Init (:67-70)."""
+    return Material(law=LAW_GUCCIONE, params=[C * a_scale, bf * b_scale, bt * b_scale, bfs * b_scale, K],
+                    density=density, **kw)
+
+
+def holzapfel(a, b, af, bf, as_, bs, afs, bfs, kappa, k=100.0, density=1082.0, **kw) -> Material:
+    return Material(law=LAW_HOLZAPFEL, params=[a, b, af, bf, as_, bs, afs, bfs, kappa, k], density=density, **kw)
+
+
+def function_params(kind: str, start=0.0, stop=0.0, keep_max=True, omega=0.0, phi=0.0,
+                    period=0.8, contr_offset=0.215, relax_offset=0.362, contr_rate=1.32, relax_rate=21.9,
+                    onset=0.0):
+    fn = [start, stop, 1.0 if keep_max else 0.0, omega, phi, period, contr_offset, relax_offset,
+          contr_rate, relax_rate, onset, 0.0]
+    if kind == "DoubleHill":
+        fn[11] = doublehill_max(fn)
+    return _FN_NAMES[kind], fn
+
+
+def doublehill_max(fn) -> float:
+    """Normalisation of DoubleHill: max over 1001 samples of one period (CBAnalyticFunction.h:155-165)."""
+    best = -math.inf
+    for i in range(1001):
+        t = i * fn[5] / 1000
+        c = (t / fn[6]) ** fn[8]
+        c = c / (1.0 + c)
+        r = (t / fn[7]) ** fn[9]
+        r = 1.0 / (1.0 + r)
+        best = max(best, c * r)
+    return best
+
+
+def analytic(kind: int, fn, time: float, prev_time: float | None = None) -> float:
+    """Host evaluation of an analytic time function (CBAnalyticFunction.h:53-145)."""
+    start, stop = fn[0], fn[1]
+    if kind == FN_LINEAR:
+        prev = time if prev_time is None else prev_time
+        if time < start:
+            return 0.0
+        if time < stop + 0.5 * (time - prev):
+            return (time - start) * 1.0 / (stop - start)
+        return 1.0 if fn[2] != 0.0 else 0.0
+    if kind == FN_SINUS:
+        return 0.5 - 0.5 * math.cos(fn[3] * (time - start) + fn[4]) if start <= time <= stop else 0.0
+    if kind == FN_SINUSDRIVER:
+        return math.sin(3.14159 * (time - start) / (stop - start)) if start <= time <= stop else 0.0
+    if kind == FN_DOUBLEHILL:
+        if not (start <= time <= stop):
+            return 0.0
+        t = time - fn[10]
+        t = math.fmod(t, fn[5]) if t > 0.0 else 0.0
+        c = (t / fn[6]) ** fn[8]
+        c = c / (1.0 + c)
+        r = (t / fn[7]) ** fn[9]
+        r = 1.0 / (1.0 + r)
+        return c * r / (fn[11] if fn[11] > 0 else doublehill_max(fn))
+    return 0.0
+
+
+def _get(params: dict, mat_index, key, default=None):
+    own = f"Materials.Mat_{mat_index}.{key}"
+    dflt = f"Materials.Mat_Default.{key}"
+    if own not in params and dflt in params:
+        return params[dflt]
+    if own in params:
+        return params[own]
+    if default is None:
+        raise KeyError(own)
+    return default
+
+
+def from_parameters(params: dict, mat_index) -> Material:
+    """Build a Material from dotted reference keys (Appendix of the settings files)."""
+    law = str(_get(params, mat_index, "Type", "Guccione"))
+    density = float(_get(params, mat_index, "Density", 0.0))
+    ignore = int(bool(_get(params, mat_index, "IgnoreCorruptElements", False)))
+    if law == "Guccione":
+        m = guccione(*(float(_get(params, mat_index, f"Guccione.{k}")) for k in ("C", "bf", "bt", "bfs", "K")),
+                     density=density, a_scale=float(_get(params, mat_index, "Guccione.aScale", 1.0)),
+                     b_scale=float(_get(params, mat_index, "Guccione.bScale", 1.0)), ignore_corrupt=ignore)
+    elif law == "Holzapfel":
+        vals = [float(_get(params, mat_index, f"Holzapfel.{k}")) for k in
+                ("a", "b", "af", "bf", "as", "bs", "afs", "bfs", "kappa")]
+        m = holzapfel(*vals, k=float(_get(params, mat_index, "Holzapfel.k", 100.0)), density=density,
+                      ignore_corrupt=ignore)
+    else:
+        raise ValueError(f"material law {law} is not on this path (Guccione, Holzapfel)")
+    tmodel = str(_get(params, mat_index, "TensionModel", "None"))
+    if tmodel == "FromFunction":
+        # Tmax has no Mat_Default fallback (CBTensionFromFunction.cpp:44)
+        m.tension_max = float(params[f"Materials.Mat_{mat_index}.TensionMax"])
+        base = f"Materials.Mat_{mat_index}.FromFunction"
+        kind = str(_get(params, mat_index, "FromFunction.Type", "None"))
+        g = lambda k, d: float(params.get(f"{base}.{k}", d))  # noqa: E731
+        m.tension_fn, m.fn = function_params(
+            kind, start=g("StartTime", 0.0), stop=g("StopTime", 0.0),
+            keep_max=bool(params.get(f"{base}.KeepMaxValue", True)), omega=g("Sinus.Omega", 0.0),
+            phi=g("Sinus.Phi", 0.0), period=g("DoubleHill.Period", 0.8),
+            contr_offset=g("DoubleHill.ContrTimeOffset", 0.215), relax_offset=g("DoubleHill.RelaxTimeOffset", 0.362),
+            contr_rate=g("DoubleHill.ContrRateConst", 1.32), relax_rate=g("DoubleHill.RelaxRateConst", 21.9),
+            onset=g("DoubleHill.OnsetTime", 0.0))
+    return m
+
+
+# ---- the workloads' materials (SI units) -------------------------------------
+
+def cube_guccione() -> Material:
+    """Unit-cube benchmark: C=0.88 kPa, bf=8, bt=6, bfs=12, kappa=1e3 kPa."""
+    return guccione(880.0, 8.0, 6.0, 12.0, 1e6, density=1082.0)
+
+
+def lv_holzapfel(tension_max=60e3, twitch=0.3) -> Material:
+    """LV beat: Holzapfel-Ogden a=0.33 kPa, b=9.24, af=18.5 kPa, bf=15.97, as=2.56 kPa, bs=10.45,
+    afs=0.42 kPa, bfs=11.6, kappa=1e3 kPa, k=100; active Tmax * sin^2 (Sinus over one twitch)."""
+    m = holzapfel(330.0, 9.24, 18500.0, 15.97, 2560.0, 10.45, 420.0, 11.6, 1e6, k=100.0, density=1082.0)
+    m.tension_max = tension_max
+    m.tension_fn, m.fn = function_params("Sinus", start=0.0, stop=twitch, omega=2 * math.pi / twitch, phi=0.0)
+    return m

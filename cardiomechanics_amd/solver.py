@@ -1,0 +1,184 @@
+"""Time loop, adaptive step control and the prescribed-pressure plugin (host side).
+
+  Timing                    CBTiming (CBTiming.cpp:25-162): halve dt on FAILED, double it
+                            after Solver.MinSteps successes, stop below Solver.MinTimeStep
+  ApplyPressureFromFunction CBApplyPressureFromFunction (CBApplyPressureFromFunction.cpp:20-132):
+                            per surface group, piecewise intervals of an analytic function,
+                            pressure = Offset + Amplitude * g(t - Offset) on [start, stop + dt/2)
+  run                       CBSolver::Run (CBSolver.cpp:1724-1858): a first pass at t = start,
+                            then successful steps until StopTime; REPEAT retries, FAILED halves dt
+The solver steps themselves (static Newton, Newmark) run on the GPU in libcmg.so.
+"""
+from __future__ import annotations
+
+import math
+import time as _time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .materials import FN_LINEAR, FN_SINUS, analytic
+
+
+class Timing:
+    def __init__(self, start=0.0, stop=1.0, dt=1e-4, min_dt=1e-9, max_dt=None, max_init_dt=None, min_steps=4,
+                 fast_relaxation=False):
+        self.start, self.stop = start, stop
+        self.first_dt = dt
+        self.min_dt = min_dt
+        self.max_sim_dt = dt if max_dt is None else max_dt
+        self.max_init_dt = self.max_sim_dt if max_init_dt is None else max_init_dt
+        self.min_steps = min_steps
+        self.fast_relaxation = fast_relaxation
+        self.reset_for_init()
+
+    def reset_for_init(self):
+        self.max_dt = self.max_init_dt
+        self.dt = min(self.first_dt, self.max_dt)
+        self.counter = 0
+        self.lowest_level_counter = 0
+
+    def reset_for_sim(self):
+        self.max_dt = self.max_sim_dt
+        self.dt = min(self.first_dt, self.max_dt)
+        self.counter = 0
+        self.lowest_level_counter = 0
+
+    def reset_counter(self):
+        self.counter = 0
+        self.lowest_level_counter = 0
+
+    def is_valid(self, t):
+        return t < self.stop and self.stop - t > 1e-14
+
+    def compute_next(self, status) -> bool:
+        """CBTiming::ComputeNextTimeStep (:77-136); False when dt fell below the minimum."""
+        if status == _lib.FAILED:
+            self.lowest_level_counter = 0
+            self.counter *= 2
+            self.dt /= 2
+            return self.dt >= self.min_dt
+        if status == _lib.SUCCESS:
+            self.counter += 1
+            self.lowest_level_counter += 1
+            while (self.counter % 2 == 0 and self.counter % self.min_steps == 0
+                   and self.lowest_level_counter >= self.min_steps):
+                self.counter //= 2
+                if self.dt < self.max_init_dt:
+                    self.dt *= 2
+                    if not self.fast_relaxation:
+                        break
+            return True
+        return True   # REPEAT: same dt
+
+
+@dataclass
+class Interval:
+    kind: int
+    start: float
+    stop: float
+    amplitude: float
+    offset: float = 0.0
+    invert: bool = False
+    fn: list = field(default_factory=lambda: [0.0] * 12)
+    prev_time: float = 0.0
+
+
+class ApplyPressureFromFunction:
+    """groups: {group index: [Interval, ...]} (group index = position in the context's group table)."""
+
+    def __init__(self, groups: dict):
+        self.groups = groups
+        self.pressure = {g: 0.0 for g in groups}
+
+    def apply(self, time, dt):
+        for g, intervals in self.groups.items():
+            for iv in intervals:
+                if iv.start <= time < iv.stop + 0.5 * dt:
+                    tt = time - iv.offset                                  # CBDataFromFunction.cpp:45-49
+                    f = analytic(iv.kind, iv.fn, tt, iv.prev_time)
+                    iv.prev_time = tt
+                    self.pressure[g] = iv.offset + iv.amplitude * ((1.0 - f) if iv.invert else f)
+                    break
+        return self.pressure
+
+    def values(self, n_groups):
+        p = np.zeros(n_groups)
+        for g, v in self.pressure.items():
+            p[g] = v
+        return p
+
+
+def linear_ramp(amplitude, start=0.0, stop=1.0, group=0):
+    fn = [start, stop, 1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0]
+    return ApplyPressureFromFunction({group: [Interval(FN_LINEAR, start, stop, amplitude, fn=fn)]})
+
+
+def sinus_pulse(amplitude, start=0.0, stop=0.8, group=0):
+    """Amplitude * (1/2 - 1/2 cos(2 pi (t - start)/(stop - start))) on [start, stop]."""
+    w = 2 * math.pi / (stop - start)
+    fn = [start, stop, 1.0, w, 0.0, 0, 0, 0, 0, 0, 0, 0]
+    return ApplyPressureFromFunction({group: [Interval(FN_SINUS, start, stop, amplitude, fn=fn)]})
+
+
+@dataclass
+class StepRecord:
+    time: float
+    dt: float
+    status: int
+    newton: int
+    krylov: int
+    residual: float
+    wall: float
+    volumes: np.ndarray
+
+
+def run(ctx, kind: str, timing: Timing, plugin: ApplyPressureFromFunction | None = None,
+        on_step=None, warm_start=None, max_steps=None):
+    """CBSolver::Run with the GPU solver step.  kind = 'static' or 'newmark'.
+    warm_start(ctx, t, dt) may return an increment used as the Newton initial guess."""
+    records = []
+    time = timing.start
+    first = True
+    timing.reset_for_init()
+    steps = 0
+    while timing.is_valid(time) or first:
+        while True:
+            t = time + timing.dt
+            if first:
+                time = timing.start - timing.dt
+                t = timing.start
+            p = plugin.apply(t, timing.dt) if plugin else {}
+            if plugin:
+                ctx.set_pressures(plugin.values(ctx.n_groups))
+            if warm_start is not None:
+                g = warm_start(ctx, t, timing.dt)
+                if g is not None:
+                    ctx.set_initial_guess(g)
+            t0 = _time.perf_counter()
+            if kind == "static":
+                rc, st = ctx.static_step(t, timing.dt)
+            else:
+                rc, st = ctx.newmark_step(t, timing.dt)
+            wall = _time.perf_counter() - t0
+            ok = timing.compute_next(rc)
+            if rc == _lib.SUCCESS:
+                vols = ctx.cavity_volumes()
+                rec = StepRecord(t, timing.dt if first else t - time, rc, st.newton_iterations,
+                                 st.krylov_iterations, st.residual_norm, wall, vols)
+                records.append(rec)
+                if on_step:
+                    on_step(rec, st)
+                time = t
+                break
+            if not ok:
+                raise RuntimeError(f"minimum time step reached at t={t}: simulation failed")
+        if first:
+            first = False
+            timing.reset_for_sim()
+            timing.reset_counter()
+        steps += 1
+        if max_steps is not None and steps >= max_steps:
+            break
+    return records

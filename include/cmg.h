@@ -1,0 +1,217 @@
+/*
+ * cmg.h -- C ABI of the MI355X (gfx950) mechanics hot path.
+ *
+ * One context owns every device buffer of one mesh on one GPU: coordinates,
+ * connectivity, reference shape-function derivatives, fibre bases, the fixed
+ * 3x3-block sparsity pattern, the deterministic assembly maps and the solver
+ * workspace.  Plain pointers and sizes only; no exceptions cross this ABI.
+ * Return values are CBStatus codes (mechanics/src/CBStatus/CBStatus.h:20-37)
+ * or the negative CMG_ERR_* argument/runtime codes below.
+ *
+ * Conventions (mechanics/src/CBElement/CBElementAdapter.h:55-60): 0-based node
+ * ids, DOF = 3*node + component, coordinates in metres, SI units throughout.
+ * Host-pointer entry points copy; *_device entry points take device pointers
+ * and run stream-ordered on the context's stream (they do not synchronise).
+ * Calls on one context are not re-entrant.
+ */
+#ifndef CMG_H
+#define CMG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMG_API_VERSION 1
+
+/* CBStatus values */
+#define CMG_SUCCESS 2
+#define CMG_NOTHING_DONE 0
+#define CMG_REPEAT (-2)
+#define CMG_FAILED (-3)
+#define CMG_INFINITIVE (-4)
+#define CMG_NOT_A_NUMBER (-5)
+#define CMG_CORRUPT_ELEMENT (-6)
+#define CMG_SYSTEM_WRONG_STATE (-8)
+/* ABI-level errors */
+#define CMG_ERR_INVALID (-100)
+#define CMG_ERR_HIP (-101)
+#define CMG_ERR_NOMEM (-102)
+
+#define CMG_LAW_GUCCIONE 0
+#define CMG_LAW_HOLZAPFEL 1
+
+#define CMG_FN_NONE 0
+#define CMG_FN_LINEAR 1
+#define CMG_FN_SINUS 2
+#define CMG_FN_SINUSDRIVER 3
+#define CMG_FN_DOUBLEHILL 4
+
+#define CMG_PREC_FP64 0
+#define CMG_PREC_MIXED 1
+
+typedef struct cmg_ctx cmg_ctx;
+
+/* One material: CBMaterial (CBMaterial.cpp:19-74) + its constitutive model
+ * (CBConstitutiveModelGuccione.cpp:20-71 / CBConstitutiveModelHolzapfel.cpp:18-85)
+ * + the FromFunction tension model (CBTensionFromFunction.cpp:36-56).
+ *   Guccione  params: C, bf, bt, bfs, K          (aScale/bScale already applied)
+ *   Holzapfel params: a, b, af, bf, as, bs, afs, bfs, kappa, k
+ *   fn: startTime, stopTime, keepMaxValue, omega, phi, DoubleHill period,
+ *       contrTimeOffset, relaxTimeOffset, contrRateConst, relaxRateConst,
+ *       onsetTime, driverMax                                                  */
+typedef struct cmg_material {
+    int32_t law;
+    int32_t ignore_corrupt;   /* Materials.Mat_<id>.IgnoreCorruptElements */
+    int32_t tension_fn;       /* CMG_FN_* */
+    int32_t reserved;
+    double density;           /* Materials.Mat_<id>.Density (kg/m^3) */
+    double params[12];
+    double tension_max;       /* Materials.Mat_<id>.TensionMax (Pa) */
+    double fn[12];
+} cmg_material;
+
+/* Mesh as the Tetgen loader hands it to the solver (CBModelLoaderTetgen.cpp:59-547). */
+typedef struct cmg_mesh {
+    int64_t n_nodes;
+    int64_t n_elements;
+    int64_t n_surfaces;
+    int32_t nodes_per_element;  /* 10 (T10) or 4 (T4) */
+    int32_t nodes_per_surface;  /* 3 (T3), 6 (T6), 0 without surfaces */
+    int32_t n_groups;           /* pressure groups (cavities) */
+    int32_t reserved;
+    const double* ref_coords;       /* [n_nodes*3] reference coordinates */
+    const int32_t* elements;        /* [n_elements*npe] */
+    const double* bases;            /* [n_elements*nb*9], nb = 5 (T10: centroid, QP1..4) or 1 (T4);
+                                       row-major Matrix3 with columns f, s, n */
+    const int32_t* material_index;  /* [n_elements] or NULL (all 0) */
+    const double* activation_time;  /* [n_elements] or NULL (all 0) */
+    const uint8_t* bc;              /* [n_nodes] bitmask 1=x 2=y 4=z, or NULL */
+    const int32_t* surfaces;        /* [n_surfaces*nps] */
+    const int32_t* surface_group;   /* [n_surfaces] pressure group of each surface element */
+} cmg_mesh;
+
+typedef struct cmg_options {
+    int32_t precision;          /* CMG_PREC_FP64 (default) or CMG_PREC_MIXED: FP32 tangent
+                                   values, FP32 preconditioned Krylov operator; residuals,
+                                   det J / penalty and convergence checks stay FP64 */
+    int32_t transpose_tangent;  /* 1: assemble each T10 element tangent transposed, the
+                                   reference convention (CBElementSolidT10.cpp:559-565) */
+    int32_t pressure_tangent;   /* 0: none (the reference's T3 tangent is zero,
+                                   CBElementSurfaceT3.cpp:268-269); 1: exact (default) */
+    int32_t consistent_mass;    /* Solver.NewmarkBeta.ConsistentMassMatrix (default 1) */
+    int32_t newton_maxit;       /* Solver.MaxIterations (default 50) */
+    int32_t krylov_restart;     /* GMRES restart length (default 80) */
+    int32_t krylov_maxit;       /* Krylov iterations per linear solve (default 4000) */
+    int32_t device;             /* HIP device ordinal (default 0) */
+    double newton_atol;         /* Solver.Precision used as SNES abstol/rtol/stol (default 1e-8) */
+    double newton_rtol;
+    double newton_stol;
+    double krylov_rtol;         /* linear-solve relative tolerance (default 1e-6) */
+    double newmark_beta;        /* Solver.NewmarkBeta.Beta  (default 0.25) */
+    double newmark_gamma;       /* Solver.NewmarkBeta.Gamma (default 0.5) */
+    double fd_epsilon;          /* Solver.Epsilon (default 1e-12), reported only */
+    double reserved[4];
+} cmg_options;
+
+typedef struct cmg_solve_stats {
+    int32_t iterations;
+    int32_t converged;
+    double initial_residual;
+    double final_residual;
+} cmg_solve_stats;
+
+typedef struct cmg_step_stats {
+    int32_t newton_iterations;   /* SNESGetIterationNumber */
+    int32_t krylov_iterations;   /* KSPGetTotalIterations */
+    int32_t converged_reason;    /* >0 converged (2 atol, 3 rtol, 4 stol), <0 diverged */
+    int32_t corrupt_element;     /* first element with det F <= 0, or -1 */
+    double residual_norm;        /* final ||r||_2 */
+    double initial_residual_norm;
+    double step_norm;
+    double wall_seconds;
+    double tangent_seconds;
+    double krylov_seconds;
+} cmg_step_stats;
+
+void cmg_default_options(cmg_options* opt);
+const char* cmg_last_error(void);
+int cmg_device_count(void);
+
+/* Replaces CBSolver::Init's mesh distribution + CBSolver::PrepareElements
+ * (CBSolver.cpp:1222-1275, 2157-2168) and CBModel::CalcNonZeroes (CBModel.cpp:43-83):
+ * uploads the mesh, computes dN/dX, the block pattern and the assembly maps. */
+int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_materials,
+               const cmg_options* options, cmg_ctx** out);
+void cmg_destroy(cmg_ctx* ctx);
+
+/* sizes: n_nodes, n_elements, n_dofs, number of nonzero 3x3 blocks */
+int cmg_sizes(cmg_ctx* ctx, int64_t* n_nodes, int64_t* n_elements, int64_t* n_dofs, int64_t* nnz_blocks);
+/* The fixed block-CSR pattern: rowptr [n_nodes+1], colind [nnz_blocks]; values of
+ * every matrix of this ABI are [nnz_blocks][3][3] row-major blocks. */
+int cmg_pattern(cmg_ctx* ctx, int64_t* rowptr, int32_t* colind);
+int cmg_set_pressures(cmg_ctx* ctx, const double* pressures /* [n_groups] */);
+int cmg_synchronize(cmg_ctx* ctx);
+/* opaque stream handle (hipStream_t) the *_device entry points run on */
+void* cmg_stream(cmg_ctx* ctx);
+
+/* ---- Seam 1: element loop, CBFormulation::CalcNodalForces /
+ *      CalcNodalForcesJacobian (CBFormulation.h:30-41, CBFormulationTotalLagrangian.cpp:18-40) */
+
+/* Internal nodal forces of all solid elements at current coordinates x [3N],
+ * summed per DOF, BC components zero (CBElementSolidT10.cpp:620-693 + adapter
+ * AddNodalForcesComponents, CBElementAdapter.cpp:138-141).  Surface pressure
+ * forces of the current pressures are added when include_pressure != 0
+ * (CBCirculationCavity::ApplyPressure, CBCirculationCavity.cpp:108-172). */
+int cmg_residual(cmg_ctx* ctx, const double* x, double time, int32_t include_pressure, double* f);
+/* Assembled tangent d f / d x in the block pattern, identity on constrained DOFs
+ * (CBSolver.cpp:413-435), values [nnz_blocks*9] (double; float in mixed precision). */
+int cmg_tangent(cmg_ctx* ctx, const double* x, double time, int32_t include_pressure, void* values);
+/* per-element outputs for parity checks: fe [E][3npe]; Ke [E][3npe][3npe] with
+ * Ke[r][c] = d fe_r / d x_c (no BC masking; double or float per precision) */
+int cmg_element_residuals(cmg_ctx* ctx, const double* x, double time, double* fe);
+int cmg_element_tangents(cmg_ctx* ctx, const double* x, double time, void* Ke);
+/* surface pressure forces [3N] at x for the given group pressures, and the
+ * cavity volume per group about the origin (CBCirculationCavity::CalcVolume, :77-94) */
+int cmg_surface_forces(cmg_ctx* ctx, const double* x, const double* pressures, double* f);
+int cmg_cavity_volumes(cmg_ctx* ctx, const double* x, double* volumes);
+/* consistent (or lumped) mass matrix in the block pattern
+ * (CBElementSolidT10.cpp:352-422, CBSolverNewmarkBeta.cpp:704-787) */
+int cmg_mass_matrix(cmg_ctx* ctx, double* values);
+
+/* device-pointer variants (stream ordered) */
+int cmg_residual_device(cmg_ctx* ctx, const double* x_dev, double time, int32_t include_pressure, double* f_dev);
+int cmg_residual_tangent_device(cmg_ctx* ctx, const double* x_dev, double time, int32_t include_pressure,
+                                double* f_dev, void* values_dev);
+/* element kernel only: per-element residuals and tangents into the context's
+ * element buffers (the unit measured as element residual+tangent throughput) */
+int cmg_element_kernel_device(cmg_ctx* ctx, const double* x_dev, double time, int32_t with_tangent);
+
+/* ---- linear algebra on the context's pattern ---------------------------- */
+int cmg_spmv(cmg_ctx* ctx, const double* values, const double* x, double* y);
+/* GMRES(m), right preconditioned with 3x3 nodal block Jacobi; replaces the
+ * KSP/PC of CBSolverNewmarkBeta.cpp:166-196 (a Krylov solve instead of LU). */
+int cmg_linear_solve(cmg_ctx* ctx, const double* values, const double* b, double* x, double rtol,
+                     cmg_solve_stats* stats);
+
+/* ---- Seam 2: solver step, CBSolver::SolverStep (CBSolver.h:214) ----------- */
+/* Current node coordinates (CBSolver nodes_, the configuration at the start of the
+ * next step), velocity and acceleration [3N] each; NULL leaves a field unchanged. */
+int cmg_set_state(cmg_ctx* ctx, const double* nodes, const double* velocity, const double* acceleration);
+int cmg_get_state(cmg_ctx* ctx, double* nodes, double* velocity, double* acceleration, double* displacement);
+/* Initial Newton guess for the next step's increment (overrides the predictor /
+ * rescaled-previous-increment guess once). */
+int cmg_set_initial_guess(cmg_ctx* ctx, const double* increment);
+/* CBSolverEquilibrium::SolverStep (CBSolverEquilibrium.cpp:265-363): quasi-static
+ * Newton for the increment at time t with step dt; on success nodes += increment. */
+int cmg_static_step(cmg_ctx* ctx, double time, double dt, cmg_step_stats* stats);
+/* CBSolverNewmarkBeta::SolverStep (CBSolverNewmarkBeta.cpp:829-1019): predictor,
+ * Newton on M(d - d~)/(beta dt^2) + f_int + f_p = 0, corrector. */
+int cmg_newmark_step(cmg_ctx* ctx, double time, double dt, cmg_step_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMG_H */

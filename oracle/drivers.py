@@ -1,0 +1,192 @@
+"""CPU FP64 solver drivers -- TEST INFRASTRUCTURE ONLY.
+
+Restates the reference's static and Newmark solver steps on the CPU with the
+oracle's element routines and a direct sparse solve (scipy's SuperLU, standing in
+for the reference's MUMPS LU, CBSolverNewmarkBeta.cpp:176-192):
+
+  CpuMechanics.residual   CBSolverEquilibrium::CalcNodalForces (CBSolverEquilibrium.cpp:44-108)
+                          + CBCirculationCavity::ApplyPressure (CBCirculationCavity.cpp:108-172)
+  CpuMechanics.tangent    exact (complex-step) element Jacobians assembled with the
+                          identity on constrained DOFs (CBSolver.cpp:413-435)
+  static_step             CBSolverEquilibrium::SolverStep / UpdateInitialGuess (:234-363)
+  newmark_step            CBSolverNewmarkBeta::SolverStep (CBSolverNewmarkBeta.cpp:829-1019):
+                          predictor, Newton on M(d - d~)/(beta dt^2) + f, corrector
+Newton iterates to a residual far below the GPU path's tolerance so that both
+converge to the same root.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle as O
+
+
+class CpuMechanics:
+    def __init__(self, mesh, materials, pressure_tangent=True, consistent_mass=True):
+        self.mesh = mesh
+        self.materials = list(materials)
+        self.ed = O.ElementData(mesh)
+        self.N = mesh.n_nodes
+        self.n = 3 * self.N
+        npe = mesh.npe
+        self.dofs = (3 * mesh.elements[:, :, None] + np.arange(3)).reshape(mesh.n_elements, 3 * npe)
+        self.bcdof = ((mesh.bc[:, None] >> np.arange(3)) & 1).astype(bool).ravel()
+        self.pressure_tangent = pressure_tangent
+        self.sdofs = None
+        if mesh.surfaces.shape[0]:
+            nps = mesh.surfaces.shape[1]
+            self.sdofs = (3 * mesh.surfaces[:, :, None] + np.arange(3)).reshape(-1, 3 * nps)
+            self.group_ids = np.unique(mesh.surface_id)
+            self.sgroup = np.searchsorted(self.group_ids, mesh.surface_id)
+        self.pressures = np.zeros(max(1, len(getattr(self, "group_ids", [0]))))
+        self.M = self._mass(consistent_mass)
+        self.nodes = mesh.coords.ravel().copy()
+        self.vel = np.zeros(self.n)
+        self.acc = np.zeros(self.n)
+        self.d = np.zeros(self.n)
+        self.time_last = 0.0
+        self.step_last = 0.0
+        self.guess = np.zeros(self.n)
+
+    # -- element / surface loops ------------------------------------------------
+    def _mass(self, consistent):
+        rows, cols, vals = [], [], []
+        for e in range(self.mesh.n_elements):
+            bc = self.bcdof[self.dofs[e]].astype(np.uint8)
+            rho = self.materials[self.mesh.material[e]].density
+            if self.mesh.npe == 10:
+                detJ = self.ed.w[e]
+                Me = O.t10_mass_consistent(detJ, rho, bc) if consistent else O.t10_mass_lumped(detJ / 6, rho, bc)
+            else:
+                Me = O.t4_mass_consistent(self.ed.w[e], rho, bc)
+            rows.append(np.repeat(self.dofs[e], Me.shape[0]))
+            cols.append(np.tile(self.dofs[e], Me.shape[0]))
+            vals.append(Me.ravel())
+        return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                             shape=(self.n, self.n))
+
+    def surface_forces(self, x):
+        f = np.zeros(self.n)
+        if self.sdofs is None:
+            return f
+        xs = x[self.sdofs]
+        bcs = self.bcdof[self.sdofs].astype(np.uint8)
+        for s in range(xs.shape[0]):
+            np.add.at(f, self.sdofs[s], O.surface_forces(xs[s], bcs[s], self.pressures[self.sgroup[s]]))
+        return f
+
+    def residual(self, x, time, include_pressure=True):
+        rc, fe, st = O.residual_batch(self.ed, x.reshape(-1, 3), self.materials, time)
+        if rc != O.SUCCESS:
+            raise FloatingPointError(f"corrupt element {int(np.argmax(st != O.SUCCESS))}")
+        f = np.zeros(self.n)
+        np.add.at(f, self.dofs.ravel(), fe.ravel())
+        if include_pressure:
+            f += self.surface_forces(x)
+        f[self.bcdof] = 0.0
+        return f
+
+    def element_tangents(self, x, time):
+        rc, Ke, st = O.tangent_batch(self.ed, x.reshape(-1, 3), self.materials, time, kind="cs")
+        if rc != O.SUCCESS:
+            raise FloatingPointError("corrupt element")
+        return Ke
+
+    def tangent(self, x, time, include_pressure=True, cm=0.0):
+        Ke = self.element_tangents(x, time)
+        nd = Ke.shape[1]
+        rows = np.repeat(self.dofs, nd, axis=1).ravel()
+        cols = np.tile(self.dofs, (1, nd)).ravel()
+        vals = Ke.ravel()
+        if include_pressure and self.pressure_tangent and self.sdofs is not None:
+            xs = x[self.sdofs]
+            bcs = self.bcdof[self.sdofs].astype(np.uint8)
+            ns = self.sdofs.shape[1]
+            Ks = np.array([O.surface_cs_tangent(xs[s], bcs[s], self.pressures[self.sgroup[s]])
+                           for s in range(xs.shape[0])])
+            rows = np.concatenate([rows, np.repeat(self.sdofs, ns, axis=1).ravel()])
+            cols = np.concatenate([cols, np.tile(self.sdofs, (1, ns)).ravel()])
+            vals = np.concatenate([vals, Ks.ravel()])
+        K = sp.csr_matrix((vals, (rows, cols)), shape=(self.n, self.n))
+        if cm != 0.0:
+            K = K + cm * self.M
+        return self.apply_bc(K)
+
+    def apply_bc(self, K):
+        K = K.tocsr()
+        free = (~self.bcdof).astype(float)
+        D = sp.diags(free)
+        K = D @ K @ D + sp.diags(self.bcdof.astype(float))
+        return K.tocsr()
+
+    # -- Newton -----------------------------------------------------------------
+    def newton(self, time, d, cm=0.0, dpred=None, tol=1e-13, maxit=40):
+        """Full Newton with a direct solve; converges until ||r|| <= tol * max(1, ||r0||)."""
+        r0 = None
+        for it in range(maxit + 1):
+            x = self.nodes + d
+            r = self.residual(x, time)
+            if cm != 0.0:
+                inert = cm * (self.M @ (d - dpred))
+                inert[self.bcdof] = 0.0
+                r = r + inert
+            rn = np.linalg.norm(r)
+            if r0 is None:
+                r0 = rn
+            if rn <= tol * max(r0, 1e-300) or rn == 0.0:
+                return d, it, rn
+            K = self.tangent(x, time, cm=cm)
+            delta = spla.spsolve(K.tocsc(), -r)
+            lam = 1.0
+            while True:
+                dn = d + lam * delta
+                try:
+                    rr = self.residual(self.nodes + dn, time)
+                    if cm != 0.0:
+                        ii = cm * (self.M @ (dn - dpred))
+                        ii[self.bcdof] = 0.0
+                        rr = rr + ii
+                    if np.linalg.norm(rr) <= rn or lam < 1.0 / 64:
+                        break
+                except FloatingPointError:
+                    pass
+                lam *= 0.5
+            d = dn
+        raise RuntimeError(f"oracle Newton did not converge: |r|={rn:.3e} r0={r0:.3e}")
+
+    def static_step(self, time, dt, tol=1e-13):
+        if time > self.time_last:
+            if self.step_last > 0:
+                self.guess = self.d / self.step_last
+                self.d = self.d * (dt / self.step_last)
+            else:
+                self.d = np.zeros(self.n)
+        elif time == self.time_last:
+            self.d = np.zeros(self.n)
+        else:
+            self.d = self.guess * dt
+        self.time_last, self.step_last = time, dt
+        self.d, it, rn = self.newton(time, self.d, tol=tol)
+        self.nodes = self.nodes + self.d
+        return it, rn
+
+    def newmark_step(self, time, dt, beta=0.25, gamma=0.5, tol=1e-13):
+        dpred = dt * self.vel + dt * dt * 0.5 * (1 - 2 * beta) * self.acc
+        vpred = 1.0 * self.vel + (1 - gamma) * dt * self.acc
+        cm = 1.0 / (beta * dt * dt)
+        d, it, rn = self.newton(time, dpred.copy(), cm=cm, dpred=dpred, tol=tol)
+        self.acc = 1.0 / (beta * dt * dt) * d + -1.0 / (beta * dt * dt) * dpred
+        self.vel = vpred + gamma * dt * self.acc
+        self.d = d
+        self.nodes = self.nodes + d
+        return it, rn
+
+    def cavity_volumes(self, x=None):
+        x = self.nodes if x is None else x
+        vols = np.zeros(len(self.group_ids))
+        xs = x[self.sdofs]
+        for s in range(xs.shape[0]):
+            vols[self.sgroup[s]] += O.surface_volume(xs[s])
+        return vols
