@@ -44,6 +44,7 @@ __device__ double dh_raw(const double* fn, double t) {
 }
 
 __device__ double analytic(int type, const double* fn, double time) {
+#pragma clang fp contract(off)
     const double start = fn[0], stop = fn[1];
     switch (type) {
     case CMG_FN_LINEAR:
@@ -78,6 +79,7 @@ __device__ __forceinline__ double tension_of(const DevMaterial& m, double time, 
 // 3x3 helpers, row-major, reference term order (Matrix3.h:290-303,482-540)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void mul3(const double* a, const double* b, double* r) {
+#pragma clang fp contract(off)
     r[0] = a[0] * b[0] + a[1] * b[3] + a[2] * b[6];
     r[1] = a[0] * b[1] + a[1] * b[4] + a[2] * b[7];
     r[2] = a[0] * b[2] + a[1] * b[5] + a[2] * b[8];
@@ -96,11 +98,13 @@ __device__ __forceinline__ void tr3(const double* a, double* r) {
 }
 
 __device__ __forceinline__ double det3(const double* e) {
+#pragma clang fp contract(off)
     return (e[0] * e[4] * e[8]) + (e[3] * e[7] * e[2]) + (e[6] * e[1] * e[5]) - (e[2] * e[4] * e[6]) -
            (e[5] * e[7] * e[0]) - (e[8] * e[3] * e[1]);
 }
 
 __device__ __forceinline__ void inv3(const double* e, double* r) {
+#pragma clang fp contract(off)
     double d = det3(e);
     if (d == 0.0) {
         for (int i = 0; i < 9; i++) r[i] = INFINITY;
@@ -138,6 +142,7 @@ struct Law {
 // returns false for det F <= 0 (CBStatus::CORRUPT_ELEMENT)
 __device__ __forceinline__ bool eval_law(const DevMaterial& m, const double* F, double Ta, bool want_tangent,
                                          Law& L) {
+#pragma clang fp contract(off)
     const double J = det3(F);
     if (!m.ignore_corrupt && (J <= 0.0)) return false;
     double Ft[9], C[9];
@@ -285,6 +290,7 @@ __device__ __forceinline__ void load_g(const ElemArgs& a, int64_t e, int q, doub
 template <int NPE>
 __device__ __forceinline__ void local_F(const double* x, const double (*g)[3], const double* B, double* Binv,
                                         double* Floc) {
+#pragma clang fp contract(off)
     double F[9];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -308,6 +314,7 @@ __device__ __forceinline__ void local_F(const double* x, const double (*g)[3], c
 // ---------------------------------------------------------------------------
 template <int NPE, int NQP>
 __global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a) {
+#pragma clang fp contract(off)
     const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int64_t e = (int64_t)bid * kResidualBlock + threadIdx.x;
     if (e >= a.E) return;
@@ -409,6 +416,7 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a) {
                     G[k][r] = (TT)(Binv[3 * r] * g[k][0] + Binv[3 * r + 1] * g[k][1] + Binv[3 * r + 2] * g[k][2]);
             if (jj == 0) {
                 // residual component ii in the reference form (FP64, exact dN/dX)
+#pragma clang fp contract(off)
                 double FS[9], St[9], BinvT[9], Bt[9], P1[9], P[9];
                 mul3(Floc, L.S, FS);
                 tr3(FS, St);
@@ -612,6 +620,7 @@ __global__ __launch_bounds__(64) void k_surface(const SurfArgs a, bool tangent) 
 // dN/dX at the 4 quadrature points (SoA) and detJ/24 per element,
 // CBElementSolidT10::CalcShapeFunctionDerivatives (:94-264)
 __device__ double t10_shape_point(const double* X, double l1, double l2, double l3, double l4, double* out) {
+#pragma clang fp contract(off)
     double Jx[4], Jy[4], Jz[4];
     const double* c[10];
     for (int k = 0; k < 10; k++) c[k] = X + 3 * k;

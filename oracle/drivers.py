@@ -122,9 +122,11 @@ class CpuMechanics:
         return K.tocsr()
 
     # -- Newton -----------------------------------------------------------------
-    def newton(self, time, d, cm=0.0, dpred=None, tol=1e-13, maxit=40):
-        """Full Newton with a direct solve; converges until ||r|| <= tol * max(1, ||r0||)."""
+    def newton(self, time, d, cm=0.0, dpred=None, tol=1e-12, atol=1e-13, maxit=40):
+        """Full Newton with a direct solve until ||r|| <= max(tol ||r0||, atol) or the
+        Newton step falls below 1e-15 ||d|| (the residual's rounding floor)."""
         r0 = None
+        step = np.inf
         for it in range(maxit + 1):
             x = self.nodes + d
             r = self.residual(x, time)
@@ -135,7 +137,7 @@ class CpuMechanics:
             rn = np.linalg.norm(r)
             if r0 is None:
                 r0 = rn
-            if rn <= tol * max(r0, 1e-300) or rn == 0.0:
+            if rn <= max(tol * r0, atol) or rn == 0.0 or step <= 1e-15 * np.linalg.norm(d):
                 return d, it, rn
             K = self.tangent(x, time, cm=cm)
             delta = spla.spsolve(K.tocsc(), -r)
@@ -153,10 +155,11 @@ class CpuMechanics:
                 except FloatingPointError:
                     pass
                 lam *= 0.5
+            step = np.linalg.norm(dn - d)
             d = dn
         raise RuntimeError(f"oracle Newton did not converge: |r|={rn:.3e} r0={r0:.3e}")
 
-    def static_step(self, time, dt, tol=1e-13):
+    def static_step(self, time, dt, tol=1e-12):
         if time > self.time_last:
             if self.step_last > 0:
                 self.guess = self.d / self.step_last
@@ -172,7 +175,7 @@ class CpuMechanics:
         self.nodes = self.nodes + self.d
         return it, rn
 
-    def newmark_step(self, time, dt, beta=0.25, gamma=0.5, tol=1e-13):
+    def newmark_step(self, time, dt, beta=0.25, gamma=0.5, tol=1e-12):
         dpred = dt * self.vel + dt * dt * 0.5 * (1 - 2 * beta) * self.acc
         vpred = 1.0 * self.vel + (1 - gamma) * dt * self.acc
         cm = 1.0 / (beta * dt * dt)
