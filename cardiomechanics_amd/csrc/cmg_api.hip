@@ -184,6 +184,7 @@ void cmg_default_options(cmg_options* o) {
     o->krylov_restart = 80;
     o->krylov_maxit = 4000;
     o->device = 0;
+    o->element_variant = 2;
     o->newton_atol = 1e-8;
     o->newton_rtol = 1e-8;
     o->newton_stol = 1e-8;
@@ -226,6 +227,7 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
         c->opt = *options;
     else
         cmg_default_options(&c->opt);
+    if (c->opt.element_variant < 1 || c->opt.element_variant > 3) c->opt.element_variant = 2;
     c->device = c->opt.device;
     CMG_HIP_CHECK(hipSetDevice(c->device));
     CMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -555,7 +557,7 @@ int eval_elements(cmg_ctx* c, const double* x_dev, double time, bool tangent, bo
     launch_fill_int(1, INT_MAX, c->corrupt.as<int>(), s);
     ElemArgs a = elem_args(c, x_dev, time);
     if (tangent)
-        launch_element_residual_tangent(c->npe, c->fp32, a, s);
+        launch_element_residual_tangent(c->npe, c->fp32, a, s, c->opt.element_variant);
     else
         launch_element_residual(c->npe, a, s);
     if (pressure && c->S > 0) launch_surface(surf_args(c, x_dev, false), tangent && c->opt.pressure_tangent, s);
@@ -765,7 +767,7 @@ int cmg_element_kernel_device(cmg_ctx* c, const double* x_dev, double time, int3
     if (!c || !x_dev) return fail("cmg_element_kernel_device: null argument");
     ElemArgs a = elem_args(c, x_dev, time);
     if (with_tangent)
-        launch_element_residual_tangent(c->npe, c->fp32, a, c->stream);
+        launch_element_residual_tangent(c->npe, c->fp32, a, c->stream, c->opt.element_variant);
     else
         launch_element_residual(c->npe, a, c->stream);
     CMG_HIP_CHECK(hipGetLastError());

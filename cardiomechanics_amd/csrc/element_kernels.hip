@@ -454,6 +454,203 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// residual + tangent, shared quadrature-point stage (v2)
+//
+// Stage A: 28 lanes (7 elements x 4 quadrature points) each evaluate one point
+// -- F, the law, the reference-order residual partial, G = B^-1 dN/dX and the
+// data the tangent needs -- and park it in LDS.  Stage B: 63 lanes (7 elements x
+// 9 spatial pairs (i, j)) build M^{ij} from LDS and contract it with G into
+// K(a,b)_ij.  NSPLIT > 1 splits the node range a into NSPLIT passes so fewer
+// accumulators stay live (more passes, less register pressure).
+// ---------------------------------------------------------------------------
+constexpr int kQRec = 51;   // Ft(9) BBt(9) S(9) Z(9) Y|X(9) czz codot ceq d00 d11 mfs
+
+template <int NPE, int NQP, typename TT, int NSPLIT>
+__global__ __launch_bounds__(64) void k_residual_tangent_v2(const ElemArgs a) {
+    constexpr int EPW = kTanElemsPerWave;
+    __shared__ double sT[EPW][NQP][3 * NPE];
+    __shared__ TT sG[EPW][NQP][NPE * 3];
+    __shared__ TT sQ[EPW][NQP][kQRec];
+    const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int64_t E = a.E;
+    const int64_t e0 = (int64_t)bid * EPW;
+    // ---- stage A ----
+    if (lane < EPW * NQP) {
+        const int slot = lane / NQP, q = lane - NQP * (lane / NQP);
+        const int64_t e = e0 + slot;
+        if (e < E) {
+            const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
+            const double Ta = tension_of(m, a.time, a.tact[e]);
+            double x[3 * NPE];
+#pragma unroll
+            for (int k = 0; k < NPE; k++) {
+                const int64_t n = a.conn[(int64_t)k * E + e];
+                x[3 * k] = a.x[3 * n];
+                x[3 * k + 1] = a.x[3 * n + 1];
+                x[3 * k + 2] = a.x[3 * n + 2];
+            }
+            double g[NPE][3];
+            load_g<NPE>(a, e, q, g);
+            double B[9], Binv[9], Floc[9];
+#pragma unroll
+            for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
+            local_F<NPE>(x, g, B, Binv, Floc);
+            Law L;
+            if (!eval_law(m, Floc, Ta, true, L)) atomicMin(a.corrupt, (int)e);
+            {
+#pragma clang fp contract(off)
+                double FS[9], St[9], BinvT[9], Bt[9], P1[9], P[9];
+                mul3(Floc, L.S, FS);
+                tr3(FS, St);
+                tr3(Binv, BinvT);
+                tr3(B, Bt);
+                mul3(BinvT, St, P1);
+                mul3(P1, Bt, P);
+#pragma unroll
+                for (int k = 0; k < NPE; k++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        sT[slot][q][3 * k + c] = g[k][0] * P[0 + c] + g[k][1] * P[3 + c] + g[k][2] * P[6 + c];
+            }
+#pragma unroll
+            for (int k = 0; k < NPE; k++)
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    sG[slot][q][3 * k + r] =
+                        (TT)(Binv[3 * r] * g[k][0] + Binv[3 * r + 1] * g[k][1] + Binv[3 * r + 2] * g[k][2]);
+            double Ft[9];
+            mul3(B, Floc, Ft);
+            TT* rec = sQ[slot][q];
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                rec[r] = (TT)Ft[r];
+                rec[9 + r] = (TT)(B[3 * (r / 3)] * B[3 * (r % 3)] + B[3 * (r / 3) + 1] * B[3 * (r % 3) + 1] +
+                                  B[3 * (r / 3) + 2] * B[3 * (r % 3) + 2]);
+                rec[18 + r] = (TT)L.S[r];
+                rec[27 + r] = (TT)L.Z[r];
+                rec[36 + r] = (TT)((m.law == CMG_LAW_GUCCIONE) ? L.Y[r] : L.X[r]);
+            }
+            rec[45] = (TT)L.czz;
+            rec[46] = (TT)L.codot;
+            rec[47] = (TT)((m.law == CMG_LAW_GUCCIONE) ? L.ceq : 0.0);
+            rec[48] = (TT)L.d00;
+            rec[49] = (TT)L.d11;
+            rec[50] = (TT)L.mfs;
+        }
+    }
+    __syncthreads();
+    // ---- stage B ----
+    const int slot = lane / 9;
+    const int ij = lane - 9 * slot;
+    const int ii = ij / 3, jj = ij - 3 * (ij / 3);
+    const int64_t e = e0 + slot;
+    if (slot >= EPW || e >= E) return;
+    const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
+    const bool guc = m.law == CMG_LAW_GUCCIONE;
+    TT bmat[9];
+    {
+        const TT bf = (TT)m.p[1], bt = (TT)m.p[2], bfs = (TT)m.p[3];
+        const TT bb[9] = {bf, bfs, bfs, bfs, bt, bt, bfs, bt, bt};
+#pragma unroll
+        for (int r = 0; r < 9; r++) bmat[r] = bb[r];
+    }
+    const double V = a.w[e];
+    TT* Ke = static_cast<TT*>(a.Ke) + e * (9 * NPE * NPE);
+    constexpr int AH = NPE / NSPLIT;
+#pragma unroll 1
+    for (int half = 0; half < NSPLIT; half++) {
+        TT acc[AH * NPE];
+#pragma unroll
+        for (int r = 0; r < AH * NPE; r++) acc[r] = (TT)0;
+#pragma unroll 1
+        for (int q = 0; q < NQP; q++) {
+            const TT* rec = sQ[slot][q];
+            TT u[3], w[3], Z[9], S[9], YX[9];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                u[r] = rec[3 * ii + r];
+                w[r] = rec[3 * jj + r];
+            }
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                S[r] = rec[18 + r];
+                Z[r] = rec[27 + r];
+                YX[r] = rec[36 + r];
+            }
+            const TT bbt = rec[9 + ij], czz = rec[45], codot = rec[46], ceq = rec[47];
+            const TT d00 = rec[48], d11 = rec[49], mfs = rec[50];
+            TT Zu[3], Zw[3], Xu[3], Yw[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                Zu[r] = Z[3 * r] * u[0] + Z[3 * r + 1] * u[1] + Z[3 * r + 2] * u[2];
+                Zw[r] = Z[3 * r] * w[0] + Z[3 * r + 1] * w[1] + Z[3 * r + 2] * w[2];
+                Xu[r] = YX[3 * r] * u[0] + YX[3 * r + 1] * u[1] + YX[3 * r + 2] * u[2];
+                Yw[r] = YX[3 * r] * w[0] + YX[3 * r + 1] * w[1] + YX[3 * r + 2] * w[2];
+            }
+            const TT uZw = u[0] * Zw[0] + u[1] * Zw[1] + u[2] * Zw[2];
+            TT M[9];
+#pragma unroll
+            for (int mm = 0; mm < 3; mm++)
+#pragma unroll
+                for (int P = 0; P < 3; P++)
+                    M[3 * mm + P] = bbt * S[3 * P + mm] + czz * Zu[mm] * Zw[P] +
+                                    codot * (uZw * Z[3 * mm + P] + Zu[P] * Zw[mm]);
+            if (guc) {
+                const TT hc = (TT)0.5 * ceq, oc = (TT)2.0 * ceq;
+                TT h[3];
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    h[r] = u[0] * w[0] * bmat[r] + u[1] * w[1] * bmat[3 + r] + u[2] * w[2] * bmat[6 + r];
+#pragma unroll
+                for (int mm = 0; mm < 3; mm++)
+#pragma unroll
+                    for (int P = 0; P < 3; P++)
+                        M[3 * mm + P] += hc * (u[P] * w[mm] * bmat[3 * P + mm] + ((mm == P) ? h[mm] : (TT)0)) +
+                                         oc * Xu[mm] * Yw[P];
+            } else {
+                const TT av[3] = {u[1], u[0], (TT)0}, bv[3] = {w[1], w[0], (TT)0};
+#pragma unroll
+                for (int mm = 0; mm < 3; mm++)
+#pragma unroll
+                    for (int P = 0; P < 3; P++) M[3 * mm + P] += Xu[mm] * w[P] + mfs * av[mm] * bv[P];
+                M[4] += d11 * u[1] * w[1];
+            }
+            M[0] += d00 * u[0] * w[0];
+            TT G[NPE][3];
+#pragma unroll
+            for (int k = 0; k < NPE; k++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) G[k][r] = sG[slot][q][3 * k + r];
+#pragma unroll
+            for (int b = 0; b < NPE; b++) {
+                const TT mg0 = M[0] * G[b][0] + M[1] * G[b][1] + M[2] * G[b][2];
+                const TT mg1 = M[3] * G[b][0] + M[4] * G[b][1] + M[5] * G[b][2];
+                const TT mg2 = M[6] * G[b][0] + M[7] * G[b][1] + M[8] * G[b][2];
+#pragma unroll
+                for (int aa = 0; aa < AH; aa++) {
+                    const int ga = half * AH + aa;
+                    acc[aa * NPE + b] += G[ga][0] * mg0 + G[ga][1] * mg1 + G[ga][2] * mg2;
+                }
+            }
+        }
+        const TT Vt = (TT)V;
+#pragma unroll
+        for (int r = 0; r < AH * NPE; r++) Ke[9 * (half * AH * NPE + r) + ij] = Vt * acc[r];
+    }
+    if (jj == 0) {
+        double* fe = a.fe + e * (3 * NPE);
+#pragma unroll
+        for (int k = 0; k < NPE; k++) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQP; q++) t += sT[slot][q][3 * k + ii];
+            fe[3 * k + ii] = V * t;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // surface pressure loads (CBElementSurfaceT3.cpp:257-315, CBElementSurfaceT6.cpp:248-303)
 // with exact tangents by forward-mode dual numbers, and the volume contribution
 // (CBElementSurfaceT3.cpp:151-191, CBElementSurfaceT6.cpp:191-241)
@@ -697,20 +894,31 @@ void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s) {
         hipLaunchKernelGGL((k_residual<4, 1>), dim3(nblk), dim3(kResidualBlock), 0, s, a);
 }
 
-void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s) {
+void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s, int variant) {
     if (a.E == 0) return;
     const int nblk = (int)((a.E + kTanElemsPerWave - 1) / kTanElemsPerWave);
+#define CMG_LAUNCH(K) hipLaunchKernelGGL(K, dim3(nblk), dim3(64), 0, s, a)
     if (npe == 10) {
-        if (fp32)
-            hipLaunchKernelGGL((k_residual_tangent<10, 4, float>), dim3(nblk), dim3(64), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_residual_tangent<10, 4, double>), dim3(nblk), dim3(64), 0, s, a);
+        if (variant == 1) {
+            if (fp32) CMG_LAUNCH((k_residual_tangent<10, 4, float>));
+            else CMG_LAUNCH((k_residual_tangent<10, 4, double>));
+        } else if (variant == 3) {
+            if (fp32) CMG_LAUNCH((k_residual_tangent_v2<10, 4, float, 2>));
+            else CMG_LAUNCH((k_residual_tangent_v2<10, 4, double, 2>));
+        } else {
+            if (fp32) CMG_LAUNCH((k_residual_tangent_v2<10, 4, float, 1>));
+            else CMG_LAUNCH((k_residual_tangent_v2<10, 4, double, 1>));
+        }
     } else {
-        if (fp32)
-            hipLaunchKernelGGL((k_residual_tangent<4, 1, float>), dim3(nblk), dim3(64), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_residual_tangent<4, 1, double>), dim3(nblk), dim3(64), 0, s, a);
+        if (variant == 1) {
+            if (fp32) CMG_LAUNCH((k_residual_tangent<4, 1, float>));
+            else CMG_LAUNCH((k_residual_tangent<4, 1, double>));
+        } else {
+            if (fp32) CMG_LAUNCH((k_residual_tangent_v2<4, 1, float, 1>));
+            else CMG_LAUNCH((k_residual_tangent_v2<4, 1, double, 1>));
+        }
     }
+#undef CMG_LAUNCH
 }
 
 void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s) {
