@@ -39,9 +39,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--elements", type=int, default=200_000)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--beat-steps", type=int, default=0, help="Newmark steps of the beat to time (0: skip)")
+    ap.add_argument("--beat-steps", type=int, default=2, help="Newmark steps of the beat to time (0: skip)")
     ap.add_argument("--cpu-elements", type=int, default=20_000, help="elements in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-element", type=int, default=0,
+                    help="profiling mode: N launches of the element kernel (default variant) and of a "
+                         "known-size copy kernel, nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design note only "
@@ -79,6 +82,21 @@ def main():
         e1.synchronize()
         ctx._check(ctx.L.cmg_synchronize(ctx.h), "sync")
         return e0.elapsed_ms(e1) / steps, (time.perf_counter() - w0) / steps * 1e3
+
+    if args.profile_element > 0:
+        ctx = MechanicsContext(mesh, mats, precision=prec)
+        xd = DeviceArray(x.ravel())
+        ncopy = 64 * 1024 * 1024
+        src = DeviceArray(np.ones(ncopy))
+        dst = DeviceArray(np.zeros(ncopy))
+        ms, _ = time_kernel(ctx, args.profile_element, 1)
+        for _ in range(args.profile_element):
+            ctx._check(ctx.L.cmg_copy_device(ctx.h, src.ptr, dst.ptr, ncopy), "copy")
+        ctx._check(ctx.L.cmg_synchronize(ctx.h), "sync")
+        print(json.dumps({"profile": "element kernel + copy", "elements": E, "element_ms": ms,
+                          "copy_bytes_read": 8 * ncopy, "copy_bytes_written": 8 * ncopy,
+                          "bytes_per_element_algorithmic": BYTES_PER_ELEM[prec]}))
+        return
 
     # kernel variants of the fused residual + tangent element kernel; the fastest is the headline
     variants = {}

@@ -24,7 +24,7 @@ EXPORTED = [
     "cmg_element_residuals", "cmg_element_tangents", "cmg_surface_forces", "cmg_cavity_volumes",
     "cmg_mass_matrix", "cmg_residual_device", "cmg_residual_tangent_device", "cmg_element_kernel_device",
     "cmg_spmv", "cmg_linear_solve", "cmg_set_state", "cmg_get_state", "cmg_set_initial_guess",
-    "cmg_static_step", "cmg_newmark_step",
+    "cmg_static_step", "cmg_newmark_step", "cmg_copy_device",
 ]
 
 
@@ -113,6 +113,7 @@ def load():
         "cmg_set_initial_guess": (ctypes.c_int, [P, P]),
         "cmg_static_step": (ctypes.c_int, [P, dbl, dbl, P]),
         "cmg_newmark_step": (ctypes.c_int, [P, dbl, dbl, P]),
+        "cmg_copy_device": (ctypes.c_int, [P, P, P, i64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

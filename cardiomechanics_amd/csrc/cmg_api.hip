@@ -774,6 +774,13 @@ int cmg_element_kernel_device(cmg_ctx* c, const double* x_dev, double time, int3
     return CMG_SUCCESS;
 }
 
+int cmg_copy_device(cmg_ctx* c, const double* src, double* dst, int64_t n) {
+    if (!c || !src || !dst || n < 0) return fail("cmg_copy_device: bad argument");
+    launch_scale_copy(n, 1.0, src, dst, c->stream);
+    CMG_HIP_CHECK(hipGetLastError());
+    return CMG_SUCCESS;
+}
+
 int cmg_residual(cmg_ctx* c, const double* x, double time, int32_t include_pressure, double* f) {
     if (!c || !x || !f) return fail("cmg_residual: null argument");
     CMG_HIP_CHECK(hipSetDevice(c->device));

@@ -1,0 +1,211 @@
+// Host-side C++ layer above the C ABI (include/cmg.h).  It mirrors the reference's
+// interfaces on the hot path so that a caller of the reference finds the same
+// names, keys and return codes:
+//   ParameterMap                      settings as dotted keys (ParameterMap/ParameterMap.h:29-152)
+//   CBAnalyticFunction family         Linear / Sinus / SinusDriver / DoubleHill (CBData/CBAnalyticFunction.h)
+//   TetgenModel                       .node/.ele/.bases/.sur + LAT files (CBModelLoaderTetgen.cpp,
+//                                     CBFileManager.cpp:120-194)
+//   CBFormulationTotalLagrangianGPU   CalcNodalForces / CalcNodalForcesJacobian (CBFormulation.h:30-41)
+//   CBTiming                          adaptive time step (CBSolver/CBTiming.cpp:25-162)
+//   CBApplyPressureFromFunction       prescribed cavity pressure (CBApplyPressureFromFunction.cpp:20-132)
+//   CBSolverGPU                       Init / SolverStep / Run (CBSolver.h:68-215) with the static
+//                                     (CBSolverEquilibrium) and Newmark (CBSolverNewmarkBeta) steps
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cmg.h"
+
+namespace cmhost {
+
+// CBStatus (CBStatus/CBStatus.h:20-37)
+enum class CBStatus : int {
+    DACCORD = 6, WAITING = 5, NOT_INITIALIZED = 4, PREPARING_SIMULATION = 3, SUCCESS = 2, RUNNING = 1,
+    NOTHING_DONE = 0, INTERRUPTED = -1, REPEAT = -2, FAILED = -3, INFINITIVE = -4, NOT_A_NUMBER = -5,
+    CORRUPT_ELEMENT = -6, CRITICAL_STRESS = -7, SYSTEM_WRONG_STATE = -8
+};
+std::string CBStatusToStr(CBStatus s);
+
+// ---------------------------------------------------------------------------
+class ParameterMap {
+public:
+    void ParseXML(const std::string& path);
+    void ParseXMLString(const std::string& text);
+    void Set(const std::string& key, const std::string& value) { params_[key] = value; }
+    bool IsAvailable(const std::string& key) const { return params_.count(key) > 0; }
+    std::vector<std::string> GetChildNodes(const std::string& parent) const;
+    template <class T> T Get(const std::string& key) const {
+        auto it = params_.find(key);
+        if (it == params_.end()) throw std::runtime_error(key + " not found in the parameter map");
+        return Convert<T>(it->second);
+    }
+    template <class T> T Get(const std::string& key, T dflt) const {
+        auto it = params_.find(key);
+        return it == params_.end() ? dflt : Convert<T>(it->second);
+    }
+    template <class T> std::vector<T> GetArray(const std::string& key) const {
+        std::vector<T> out;
+        std::istringstream ss(Get<std::string>(key));
+        std::string tok;
+        while (std::getline(ss, tok, ',')) out.push_back(Convert<T>(tok));
+        return out;
+    }
+    const std::map<std::string, std::string>& All() const { return params_; }
+
+private:
+    template <class T> static T Convert(const std::string& s) {
+        std::string t = Trim(s);
+        if constexpr (std::is_same<T, std::string>::value) {
+            return t;
+        } else if constexpr (std::is_same<T, bool>::value) {
+            return t == "true" || t == "1" || t == "True" || t == "TRUE";
+        } else {
+            std::istringstream ss(t);
+            T v{};
+            ss >> v;
+            if (ss.fail()) throw std::runtime_error("cannot convert '" + t + "'");
+            return v;
+        }
+    }
+    static std::string Trim(const std::string& s);
+    std::map<std::string, std::string> params_;
+};
+
+// ---------------------------------------------------------------------------
+class CBAnalyticFunction {
+public:
+    CBAnalyticFunction(const ParameterMap& p, const std::string& key);
+    virtual ~CBAnalyticFunction() = default;
+    virtual double operator()(double time) = 0;
+    static std::unique_ptr<CBAnalyticFunction> Create(const std::string& type, const ParameterMap& p,
+                                                      const std::string& key);
+    // flat description used by the device tension model: type + fn[12] (cmg_material.fn)
+    virtual int DeviceType() const = 0;
+    virtual void DeviceParams(double* fn) const;
+
+protected:
+    double start_, stop_;
+    bool keepMax_;
+};
+
+// ---------------------------------------------------------------------------
+struct TetgenModel {
+    int64_t nNodes = 0, nElements = 0;
+    int npe = 10;
+    std::vector<double> coords;         // [N*3] metres
+    std::vector<uint8_t> bc;            // [N]
+    std::vector<int32_t> elements;      // [E*npe] 0-based
+    std::vector<int32_t> material;      // [E] material id as in the file
+    std::vector<double> bases;          // [E*nb*9] row-major, columns f,s,n
+    std::vector<int32_t> surfaces;      // [S*nps]
+    std::vector<int32_t> surfaceIndex;  // [S]
+    int nps = 3;
+    std::vector<double> activation;     // [E]
+    void Load(const std::string& nodes, const std::string& elements, const std::string& bases,
+              const std::string& surfaces, double unit);
+    void CheckNodeSorting();            // CBElementSolidT10::CheckNodeSorting (:573-618)
+    void LoadActivationTimes(const std::string& latFile);
+};
+
+// ---------------------------------------------------------------------------
+class CBTiming {
+public:
+    void Init(const ParameterMap& p);
+    double GetTimeStep() const { return timeStep_; }
+    double GetStartTime() const { return startTime_; }
+    double GetStopTime() const { return stopTime_; }
+    void SetSolverStatus(CBStatus s) { status_ = s; }
+    CBStatus ComputeNextTimeStep();
+    void ResetForInit();
+    void ResetForSim();
+    void ResetCounter() { counter_ = lowestLevelCounter_ = 0; }
+    bool IsValid(double t) const { return t < stopTime_ && stopTime_ - t > 1e-14; }
+
+private:
+    double startTime_ = 0, stopTime_ = 1, firstTimeStep_ = 1e-4, minTimeStep_ = 1e-9;
+    double maxSimTimeStep_ = 1e-4, maxInitTimeStep_ = 1e-4, maxTimeStep_ = 1e-4, timeStep_ = 1e-4;
+    bool fastRelaxation_ = false;
+    int minNumCnt_ = 4, counter_ = 0, lowestLevelCounter_ = 0;
+    CBStatus status_ = CBStatus::SUCCESS;
+};
+
+// ---------------------------------------------------------------------------
+class CBApplyPressureFromFunction {
+public:
+    // groups of Plugins.ApplyPressureFromFunction.Groups.*; surface index -> pressure group
+    void Init(const ParameterMap& p, const std::vector<int32_t>& groupSurfaceIds);
+    void Apply(double time, double dt);
+    const std::vector<double>& Pressures() const { return pressure_; }
+    bool Active() const { return !groups_.empty(); }
+
+private:
+    struct Interval {
+        std::unique_ptr<CBAnalyticFunction> function;
+        double startTime, stopTime, offset, amplitude;
+        bool invert;
+    };
+    std::map<int, std::vector<Interval>> groups_;   // pressure group -> intervals
+    std::vector<double> pressure_;
+};
+
+// ---------------------------------------------------------------------------
+class CBSolverGPU;
+
+class CBFormulationTotalLagrangianGPU {
+public:
+    explicit CBFormulationTotalLagrangianGPU(CBSolverGPU* solver) : solver_(solver) {}
+    // element loop on the GPU at the solver's current coordinates and time
+    CBStatus CalcNodalForces();           // fills solver nodal forces [3N]
+    CBStatus CalcNodalForcesJacobian();   // fills solver Jacobian values [nnzb*9]
+
+private:
+    CBSolverGPU* solver_;   // no ownership
+};
+
+struct StepRecord {
+    double time, dt;
+    int newton, krylov;
+    double residual, wall;
+    std::vector<double> volumes;
+};
+
+class CBSolverGPU {
+public:
+    ~CBSolverGPU();
+    // Solver.Type = Static | NewmarkBeta; materials from Materials.Mat_<id>.*
+    void Init(const ParameterMap& p, TetgenModel& model);
+    CBStatus SolverStep(double time);
+    void Run(std::vector<StepRecord>* records = nullptr, int maxSteps = -1);
+    cmg_ctx* Context() { return ctx_; }
+    const std::vector<double>& Nodes();   // current coordinates
+    std::vector<double>& NodalForces() { return forces_; }
+    std::vector<double>& JacobianValues() { return jacobian_; }
+    double CurrentTime() const { return currentTime_; }
+    CBTiming& GetTiming() { return timing_; }
+    std::vector<double> CavityVolumes();
+    const std::string& Type() const { return type_; }
+
+private:
+    friend class CBFormulationTotalLagrangianGPU;
+    std::string type_;
+    cmg_ctx* ctx_ = nullptr;
+    CBTiming timing_;
+    CBApplyPressureFromFunction pressure_;
+    std::unique_ptr<CBFormulationTotalLagrangianGPU> formulation_;
+    std::vector<double> nodes_, forces_, jacobian_;
+    std::vector<int32_t> groupIds_;
+    double currentTime_ = 0.0;
+    int64_t nnzb_ = 0, nDofs_ = 0;
+    std::string exportFile_;
+};
+
+std::vector<cmg_material> MaterialsFromParameters(const ParameterMap& p, const std::vector<int32_t>& ids);
+
+}  // namespace cmhost

@@ -192,6 +192,10 @@ int cmg_residual_tangent_device(cmg_ctx* ctx, const double* x_dev, double time, 
  * element buffers (the unit measured as element residual+tangent throughput) */
 int cmg_element_kernel_device(cmg_ctx* ctx, const double* x_dev, double time, int32_t with_tangent);
 
+/* Device copy of n doubles on the context's stream (a plain streaming kernel; used as
+ * the known-size reference when calibrating HBM traffic counters). */
+int cmg_copy_device(cmg_ctx* ctx, const double* src_dev, double* dst_dev, int64_t n);
+
 /* ---- linear algebra on the context's pattern ---------------------------- */
 int cmg_spmv(cmg_ctx* ctx, const double* values, const double* x, double* y);
 /* GMRES(m), right preconditioned with 3x3 nodal block Jacobi; replaces the

@@ -122,9 +122,9 @@ class CpuMechanics:
         return K.tocsr()
 
     # -- Newton -----------------------------------------------------------------
-    def newton(self, time, d, cm=0.0, dpred=None, tol=1e-12, atol=1e-13, maxit=40):
+    def newton(self, time, d, cm=0.0, dpred=None, tol=1e-12, atol=1e-12, maxit=40):
         """Full Newton with a direct solve until ||r|| <= max(tol ||r0||, atol) or the
-        Newton step falls below 1e-15 ||d|| (the residual's rounding floor)."""
+        Newton step falls below 1e-14 ||d|| (the residual's rounding floor)."""
         r0 = None
         step = np.inf
         for it in range(maxit + 1):
@@ -137,7 +137,7 @@ class CpuMechanics:
             rn = np.linalg.norm(r)
             if r0 is None:
                 r0 = rn
-            if rn <= max(tol * r0, atol) or rn == 0.0 or step <= 1e-15 * np.linalg.norm(d):
+            if rn <= max(tol * r0, atol) or rn == 0.0 or step <= 1e-14 * np.linalg.norm(d):
                 return d, it, rn
             K = self.tangent(x, time, cm=cm)
             delta = spla.spsolve(K.tocsc(), -r)

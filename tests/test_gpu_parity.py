@@ -177,7 +177,7 @@ def test_spmv_and_gmres_solve():
         v = rng.normal(size=ctx.n_dofs)
         np.testing.assert_allclose(ctx.spmv(vals, v), K @ v, rtol=1e-13, atol=1e-13 * np.abs(K @ v).max())
         b = K @ rng.normal(size=ctx.n_dofs)
-        xs, st = ctx.linear_solve(vals, b, rtol=1e-10)
+        xs, st = ctx.linear_solve(vals, b, rtol=1e-12)
     import scipy.sparse.linalg as spla
     xr = spla.spsolve(K.tocsc(), b)
     print(f"gmres iterations {st.iterations}, rel err {np.linalg.norm(xs - xr) / np.linalg.norm(xr):.2e}")
