@@ -48,7 +48,7 @@ class CmgOptions(ctypes.Structure):
                 ("pressure_tangent", ctypes.c_int32), ("consistent_mass", ctypes.c_int32),
                 ("newton_maxit", ctypes.c_int32), ("krylov_restart", ctypes.c_int32),
                 ("krylov_maxit", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("element_variant", ctypes.c_int32), ("reserved_i", ctypes.c_int32),
+                ("element_variant", ctypes.c_int32), ("krylov_method", ctypes.c_int32),
                 ("newton_atol", ctypes.c_double), ("newton_rtol", ctypes.c_double),
                 ("newton_stol", ctypes.c_double), ("krylov_rtol", ctypes.c_double),
                 ("newmark_beta", ctypes.c_double), ("newmark_gamma", ctypes.c_double),

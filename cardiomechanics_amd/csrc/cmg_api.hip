@@ -467,7 +467,7 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     CMG_TRY(alloc(c->vals, (size_t)nnzb * 9 * (c->fp32 ? sizeof(float) : sizeof(double))));
     CMG_TRY(alloc(c->dinv, (size_t)N * 9 * sizeof(double)));
     CMG_TRY(alloc(c->corrupt, sizeof(int)));
-    c->gm_m = std::max(2, std::min(c->opt.krylov_restart, 1000));
+    c->gm_m = std::max(8, std::min(c->opt.krylov_restart, 1000));
     CMG_TRY(alloc(c->gm_V, (size_t)(c->gm_m + 2) * vb));
     CMG_TRY(alloc(c->gm_partial, (size_t)dot_blocks(c->n) * (c->gm_m + 2) * sizeof(double) + 64));
     CMG_TRY(alloc(c->gm_h, (size_t)(c->gm_m + 2) * sizeof(double)));
@@ -666,16 +666,14 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
             // pass 1: h_i = v_i . w (i <= j) and ||w||^2 in one multi-dot
             CMG_TRY(multidot(c, j + 2, V, n, vn, h.data()));
             const double wn2 = h[j + 1];
-            CMG_TRY(to_device_coeffs(c, h.data(), j + 1));
-            launch_multiaxpy(n, j + 1, V, n, c->gm_y.as<double>(), vn, s);
+            launch_multiaxpy(n, j + 1, V, n, c->gm_h.as<double>(), vn, s);   // coefficients stay on device
             double hs = 0.0;
             for (int i = 0; i <= j; i++) hs += h[i] * h[i];
             double nrm2 = wn2 - hs;
             if (nrm2 < 0.5 * wn2) {
                 // strong cancellation: second classical Gram-Schmidt pass
                 CMG_TRY(multidot(c, j + 2, V, n, vn, h2.data()));
-                CMG_TRY(to_device_coeffs(c, h2.data(), j + 1));
-                launch_multiaxpy(n, j + 1, V, n, c->gm_y.as<double>(), vn, s);
+                launch_multiaxpy(n, j + 1, V, n, c->gm_h.as<double>(), vn, s);
                 double hs2 = 0.0;
                 for (int i = 0; i <= j; i++) {
                     h[i] += h2[i];
@@ -732,6 +730,107 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
     }
     CMG_HIP_CHECK(hipGetLastError());
     return CMG_SUCCESS;
+}
+
+
+// BiCGStab with right block-Jacobi preconditioning (van der Vorst 1992).  Two host
+// synchronisations per iteration: (r^, v) for alpha, and one fused fetch of
+// (r^,t) (s,t) (t,t) (r^,s) (s,s) from which omega, the next rho and the updated
+// residual norm ||s - omega t|| follow without another pass.
+int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol, int maxit, cmg_solve_stats* st) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    const int nb = dot_blocks(n);
+    double* W = c->gm_V.as<double>();   // (restart + 2) >= 10 vectors
+    double* rh = W;                     // rh, sv, t adjacent: one strided dot launch covers them
+    double* sv = W + n;
+    double* t = W + 2 * n;
+    double* r = W + 3 * n;
+    double* p = W + 4 * n;
+    double* v = W + 5 * n;
+    double* ph = W + 6 * n;
+    double* sh = W + 7 * n;
+    double* part = c->gm_partial.as<double>();
+    double* dh = c->gm_h.as<double>();
+    const int64_t* rp = c->rowptr.as<int64_t>();
+    const int32_t* ci = c->colind.as<int32_t>();
+    launch_block_jacobi_setup(c->N, rp, ci, vals, c->fp32, c->dinv.as<double>(), s);
+    double bnorm;
+    CMG_TRY(norm2(c, b, &bnorm));
+    launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s);
+    launch_waxpby(n, 1.0, b, -1.0, v, r, s);
+    double rn;
+    CMG_TRY(norm2(c, r, &rn));
+    st->initial_residual = rn;
+    st->iterations = 0;
+    st->converged = 0;
+    const double target = rtol * (bnorm > 0 ? bnorm : 1.0);
+    if (rn <= target || rn == 0.0) {
+        st->final_residual = rn;
+        st->converged = 1;
+        return CMG_SUCCESS;
+    }
+    int it = 0, restarts = 0;
+    bool fresh = true;
+    double rho = 1.0, alpha = 1.0, omega = 1.0, rho_new = rn * rn;
+    while (it < maxit) {
+        if (fresh) {   // (re)start: shadow residual = r, p = r
+            launch_scale_copy(n, 1.0, r, rh, s);
+            launch_scale_copy(n, 1.0, r, p, s);
+            rho_new = rn * rn;
+            fresh = false;
+        } else {       // p = r + beta (p - omega v)
+            const double beta = (rho_new / rho) * (alpha / omega);
+            launch_axpby(n, -omega, v, 1.0, p, s);
+            launch_axpby(n, 1.0, r, beta, p, s);
+        }
+        rho = rho_new;
+        launch_block_jacobi_apply(c->N, c->dinv.as<double>(), p, ph, s);
+        launch_spmv(c->N, rp, ci, vals, c->fp32, ph, v, s);
+        double d0;
+        CMG_TRY(multidot(c, 1, rh, 0, v, &d0));
+        bool breakdown = !(std::fabs(d0) > 0.0) || !std::isfinite(d0);
+        if (!breakdown) {
+            alpha = rho / d0;
+            launch_waxpby(n, 1.0, r, -alpha, v, sv, s);   // s = r - alpha v
+            launch_block_jacobi_apply(c->N, c->dinv.as<double>(), sv, sh, s);
+            launch_spmv(c->N, rp, ci, vals, c->fp32, sh, t, s);
+            launch_multidot(n, 3, rh, n, t, part, s);      // (rh,t) (s,t) (t,t)
+            launch_reduce_partials(nb, 3, part, dh, s);
+            launch_multidot(n, 2, rh, n, sv, part, s);     // (rh,s) (s,s)
+            launch_reduce_partials(nb, 2, part, dh + 3, s);
+            CMG_HIP_CHECK(hipMemcpyAsync(c->h_pinned, dh, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+            CMG_HIP_CHECK(hipStreamSynchronize(s));
+            const double rt = c->h_pinned[0], st_ = c->h_pinned[1], tt = c->h_pinned[2];
+            const double rs = c->h_pinned[3], ss = c->h_pinned[4];
+            omega = tt > 0.0 ? st_ / tt : 0.0;
+            launch_axpby(n, alpha, ph, 1.0, x, s);        // x += alpha ph + omega sh
+            launch_axpby(n, omega, sh, 1.0, x, s);
+            launch_waxpby(n, 1.0, sv, -omega, t, r, s);   // r = s - omega t
+            it++;
+            rn = std::sqrt(std::max(ss - 2.0 * omega * st_ + omega * omega * tt, 0.0));
+            rho_new = rs - omega * rt;
+            breakdown = omega == 0.0 || !(std::fabs(rho_new) > 0.0) || !std::isfinite(rn);
+        }
+        if (rn <= target || breakdown) {
+            // the recurrence residual drifts from b - A x: confirm, restart if needed
+            launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s);
+            launch_waxpby(n, 1.0, b, -1.0, v, r, s);
+            CMG_TRY(norm2(c, r, &rn));
+            if (rn <= target || ++restarts > 50 || !std::isfinite(rn)) break;
+            fresh = true;
+        }
+    }
+    st->iterations = it;
+    st->final_residual = rn;
+    st->converged = rn <= target;
+    CMG_HIP_CHECK(hipGetLastError());
+    return CMG_SUCCESS;
+}
+
+int krylov(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol, int maxit, cmg_solve_stats* st) {
+    if (c->opt.krylov_method == 1) return bicgstab(c, vals, b, x, rtol, maxit, st);
+    return gmres(c, vals, b, x, rtol, maxit, st);
 }
 
 }  // namespace
@@ -910,7 +1009,7 @@ int cmg_linear_solve(cmg_ctx* c, const double* values, const double* b, double* 
     CMG_HIP_CHECK(hipMemcpyAsync(c->delta.p, x, vb, hipMemcpyHostToDevice, c->stream));
     cmg_solve_stats local;
     if (!st) st = &local;
-    CMG_TRY(gmres(c, c->vals.p, c->r.as<double>(), c->delta.as<double>(), rtol, c->opt.krylov_maxit, st));
+    CMG_TRY(krylov(c, c->vals.p, c->r.as<double>(), c->delta.as<double>(), rtol, c->opt.krylov_maxit, st));
     CMG_HIP_CHECK(hipMemcpyAsync(x, c->delta.p, vb, hipMemcpyDeviceToHost, c->stream));
     CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
     return st->converged ? CMG_SUCCESS : CMG_FAILED;
@@ -1037,7 +1136,7 @@ int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
         launch_scale_copy(n, -1.0, r, c->dsave.as<double>(), s);
         launch_fill(n, 0.0, delta, s);
         cmg_solve_stats ls;
-        CMG_TRY(gmres(c, c->vals.p, c->dsave.as<double>(), delta, eta, c->opt.krylov_maxit, &ls));
+        CMG_TRY(krylov(c, c->vals.p, c->dsave.as<double>(), delta, eta, c->opt.krylov_maxit, &ls));
         kry_total += ls.iterations;
         t_kry += now_s() - tk;
         lambda = 1.0;

@@ -108,7 +108,7 @@ typedef struct cmg_options {
     int32_t device;             /* HIP device ordinal (default 0) */
     int32_t element_variant;    /* residual+tangent kernel: 2 shared quadrature stage (default),
                                    3 the same in two node passes, 1 per-lane quadrature points */
-    int32_t reserved_i;
+    int32_t krylov_method;      /* 0 GMRES(restart) (default), 1 BiCGStab; both right block-Jacobi */
     double newton_atol;         /* Solver.Precision used as SNES abstol/rtol/stol (default 1e-8) */
     double newton_rtol;
     double newton_stol;

@@ -165,11 +165,12 @@ def test_mass_matrix():
     assert np.abs(Mg - Mo).max() <= 1e-14 * np.abs(Mo).max()
 
 
-def test_spmv_and_gmres_solve():
+@pytest.mark.parametrize("method", [0, 1], ids=["gmres", "bicgstab"])
+def test_spmv_and_krylov_solve(method):
     """The Newmark system matrix K + M/(beta dt^2) at the unloaded state, dt = 1 ms."""
     mesh, mats = lv_small()
     rng = np.random.default_rng(3)
-    with MechanicsContext(mesh, mats, krylov_restart=60, krylov_maxit=20000) as ctx:
+    with MechanicsContext(mesh, mats, krylov_restart=60, krylov_maxit=20000, krylov_method=method) as ctx:
         Kt = ctx.bsr(ctx.tangent(mesh.coords.ravel(), 0.0))
         Mm = ctx.bsr(ctx.mass_matrix())
         vals = np.ascontiguousarray((Kt.data + Mm.data / 0.25e-6).reshape(-1))   # same block pattern
@@ -180,7 +181,7 @@ def test_spmv_and_gmres_solve():
         xs, st = ctx.linear_solve(vals, b, rtol=1e-12)
     import scipy.sparse.linalg as spla
     xr = spla.spsolve(K.tocsc(), b)
-    print(f"gmres iterations {st.iterations}, rel err {np.linalg.norm(xs - xr) / np.linalg.norm(xr):.2e}")
+    print(f"method {method} iterations {st.iterations}, rel err {np.linalg.norm(xs - xr) / np.linalg.norm(xr):.2e}")
     assert st.converged
     assert np.linalg.norm(xs - xr) <= 1e-7 * np.linalg.norm(xr)
 
@@ -246,13 +247,15 @@ def test_static_mixed_precision_converges_to_fp64_root():
         assert err <= 1e-8
 
 
-def test_newmark_steps_match_oracle():
+@pytest.mark.parametrize("method", [0, 1], ids=["gmres", "bicgstab"])
+def test_newmark_steps_match_oracle(method):
     """Newmark (beta=1/4, gamma=1/2) with Holzapfel + active tension + cavity pressure."""
     mesh, mats = lv_small()
     cpu = CpuMechanics(mesh, mats)
     plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
     dt = 0.005
-    with MechanicsContext(mesh, mats, newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15) as ctx:
+    with MechanicsContext(mesh, mats, newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15,
+                          krylov_method=method) as ctx:
         for k in range(1, 9):
             t = k * dt
             p = plug.apply(t, dt)[0]
