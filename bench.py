@@ -48,6 +48,21 @@ def main():
     ap.add_argument("--profile-element", type=int, default=0,
                     help="profiling mode: N launches of the element kernel (default variant) and of a "
                          "known-size copy kernel, nothing else")
+    ap.add_argument("--sweep", action="store_true",
+                    help="run the batched passive-parameter sweep (Guccione sets, quasi-static LV inflation) "
+                         "instead of the element-kernel benchmark")
+    ap.add_argument("--sweep-sets", type=int, default=64)
+    ap.add_argument("--sweep-elements", type=int, default=50_000)
+    ap.add_argument("--sweep-steps", type=int, default=20)
+    ap.add_argument("--sweep-check", type=int, default=2, help="sets re-solved one at a time for the check")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="train the learned Newton warm start on jittered beats and compare a beat window "
+                         "with and without it")
+    ap.add_argument("--ws-elements", type=int, default=20_000)
+    ap.add_argument("--ws-steps", type=int, default=150, help="Newmark steps (dt = 1 ms) of each beat window")
+    ap.add_argument("--ws-train", type=int, default=5, help="training beats (Tmax +-15 %%, timing +-10 ms)")
+    ap.add_argument("--ws-epochs", type=int, default=40)
+    ap.add_argument("--ws-bf16", action="store_true", help="bfloat16 inference")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design note only "
@@ -61,6 +76,10 @@ def main():
     from cardiomechanics_amd.hip import DeviceArray, Event
 
     prec = 1 if args.precision == "mixed" else 0
+    if args.sweep:
+        return sweep(args, prec)
+    if args.warm_start:
+        return warm_start(args, prec)
     t0 = time.time()
     mesh = M.lv_mesh(args.elements)
     E = mesh.n_elements
@@ -200,6 +219,150 @@ def main():
         out["beat"] = beat
     print(json.dumps(out))
     ctx.close()
+
+
+def sweep(args, prec):
+    """Passive-parameter sweep: S Guccione sets on one LV mesh, quasi-static inflation
+    0 -> 2 kPa in `sweep_steps` load steps, all sets in the same launches; a few sets are
+    re-solved one at a time (same library, single-set path) for timing and checking."""
+    from cardiomechanics_amd import materials as MT
+    from cardiomechanics_amd import mesh as M
+    from cardiomechanics_amd.context import MechanicsContext, ParameterBatch
+
+    mesh = M.lv_mesh(args.sweep_elements)
+    sets = MT.guccione_sweep(args.sweep_sets, seed=0)
+    opts = dict(precision=prec, pressure_tangent=0, krylov_maxit=20000)
+    steps = args.sweep_steps
+    log(f"[sweep] {mesh.n_elements} T10 elements, {mesh.n_nodes} nodes, {len(sets)} Guccione sets, {steps} steps")
+    with MechanicsContext(mesh, [sets[0]], **opts) as ctx:
+        with ParameterBatch(ctx, [[m] for m in sets]) as batch:
+            t0 = time.perf_counter()
+            its = []
+            for k in range(1, steps + 1):
+                t = k / steps
+                ctx.set_pressures([-2000.0 * t])
+                rc, st = batch.static_step(t, 1.0 / steps)
+                its.append((max(s.newton_iterations for s in st), max(s.krylov_iterations for s in st)))
+                log(f"[sweep] step {k}: rc={rc} max newton {its[-1][0]} max cg {its[-1][1]} "
+                    f"{time.perf_counter() - t0:.1f} s")
+                if rc != 2:
+                    break
+            t_batch = time.perf_counter() - t0
+            xb = batch.nodes()
+            info = batch.info()
+    t_single = []
+    errs = []
+    for i in range(min(args.sweep_check, len(sets))):
+        with MechanicsContext(mesh, [sets[i]], **opts) as one:
+            t0 = time.perf_counter()
+            for k in range(1, steps + 1):
+                t = k / steps
+                one.set_pressures([-2000.0 * t])
+                rc, st = one.static_step(t, 1.0 / steps)
+            t_single.append(time.perf_counter() - t0)
+            g = one.get_state()["nodes"]
+        errs.append(float(np.linalg.norm(xb[i] - g) / np.linalg.norm(g - mesh.coords.ravel())))
+        log(f"[sweep] set {i} alone: {t_single[-1]:.1f} s, rel. difference to the batch {errs[-1]:.2e}")
+    seq = float(np.mean(t_single)) * len(sets) if t_single else None
+    out = {"metric": "passive-parameter sweep wall time", "value": t_batch, "unit": "s", "n_gpus": 1,
+           "higher_is_better": False, "dtype": "f64" if prec == 0 else "f64/f32",
+           "config": {"workload": "LV quasi-static inflation 0-2 kPa, Guccione parameter sweep",
+                      "elements": mesh.n_elements, "sets": len(sets), "steps": steps},
+           "batch_seconds": t_batch, "sequential_seconds_extrapolated": seq,
+           "speedup_vs_sequential": (seq / t_batch) if seq else None, "checked_sets": len(errs),
+           "max_rel_difference_vs_single": max(errs) if errs else None, "autotuner": info,
+           "max_newton_cg_per_step": its}
+    print(json.dumps(out))
+
+
+def warm_start(args, prec):
+    """Learned warm start on a reduced version of the warm-start workload: training beats
+    with Tmax jittered by +-15 % and activation shifted by +-10 ms (seeded), then the
+    nominal beat window with and without the network at the same Newton tolerance."""
+    from cardiomechanics_amd import materials as MT
+    from cardiomechanics_amd import mesh as M
+    from cardiomechanics_amd import solver as SV
+    from cardiomechanics_amd.context import MechanicsContext
+    from cardiomechanics_amd.warmstart import WarmStartNet
+
+    mesh = M.lv_mesh(args.ws_elements)
+    dt, steps = 1e-3, args.ws_steps
+    tol = dict(precision=prec, newton_atol=1e-8, newton_rtol=1e-8, newton_stol=1e-8)
+    log(f"[warm start] {mesh.n_elements} T10 elements, {steps} steps of {dt * 1e3:.0f} ms, "
+        f"{args.ws_train} training beats")
+
+    def window(m, act, weights=None, record=False):
+        msh = M.Mesh(coords=mesh.coords, elements=mesh.elements, bases=mesh.bases, bc=mesh.bc,
+                     material=mesh.material, activation=act, surfaces=mesh.surfaces, surface_id=mesh.surface_id)
+        plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
+        newton = krylov = kept = 0
+        wall = 0.0
+        nodes = []
+        with MechanicsContext(msh, [m], **tol) as ctx:
+            net = WarmStartNet(ctx, seed=5)
+            if weights is not None:
+                net.set_weights(weights)
+            p_prev = 0.0
+            for k in range(1, steps + 1):
+                t = k * dt
+                plug.apply(t, dt)
+                p = plug.values(ctx.n_groups)
+                ctx.set_pressures(p)
+                rate = (p[0] - p_prev) / dt
+                p_prev = p[0]
+                if record:
+                    net.record_before_step(t, rate)
+                t0 = time.perf_counter()
+                if weights is not None and k > 2:
+                    net.propose(t, rate, bf16=args.ws_bf16)
+                rc, st = ctx.newmark_step(t, dt)
+                wall += time.perf_counter() - t0
+                if rc != 2:
+                    raise RuntimeError(f"Newmark step {k} failed: {st.as_dict()}")
+                if record:
+                    net.record_after_step()
+                newton += st.newton_iterations
+                krylov += st.krylov_iterations
+                kept += st.guess_source == 2
+                nodes.append(ctx.get_state()["nodes"])
+            samples = net.samples
+            net.close()
+        return dict(newton=newton, krylov=krylov, wall=wall, kept=kept, nodes=nodes, samples=samples)
+
+    rng = np.random.default_rng(0)
+    samples = []
+    t0 = time.perf_counter()
+    for b in range(args.ws_train):
+        m = MT.lv_holzapfel(tension_max=60e3 * rng.uniform(0.85, 1.15))
+        r = window(m, mesh.activation + rng.uniform(-0.01, 0.01), record=True)
+        samples += r["samples"]
+        log(f"[warm start] training beat {b}: newton {r['newton']} krylov {r['krylov']} {r['wall']:.1f} s")
+    t_rec = time.perf_counter() - t0
+    with MechanicsContext(mesh, [MT.lv_holzapfel()], **tol) as tctx:
+        net = WarmStartNet(tctx, seed=5)
+        net.samples = samples
+        t0 = time.perf_counter()
+        hist = net.train(epochs=args.ws_epochs, lr=2e-3, log=log)
+        t_train = time.perf_counter() - t0
+        weights = net.weights()
+        net.close()
+    plain = window(MT.lv_holzapfel(), mesh.activation)
+    warm = window(MT.lv_holzapfel(), mesh.activation, weights=weights)
+    errs = [float(np.linalg.norm(a - b) / max(np.linalg.norm(a - mesh.coords.ravel()), 1e-300))
+            for a, b in zip(plain["nodes"], warm["nodes"])]
+    log(f"[warm start] plain: newton {plain['newton']} krylov {plain['krylov']} {plain['wall']:.2f} s; "
+        f"warm: newton {warm['newton']} krylov {warm['krylov']} {warm['wall']:.2f} s, "
+        f"guess kept {warm['kept']}/{steps - 2}; max rel. difference {max(errs):.2e}")
+    out = {"metric": "learned warm start: Newmark window wall time", "value": warm["wall"], "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "f64 (network f32" + ("/bf16" if args.ws_bf16 else "") + ")",
+           "config": {"workload": "LV Newmark beat window, Holzapfel-Ogden + active tension", "elements": mesh.n_elements,
+                      "steps": steps, "dt": dt, "training_beats": args.ws_train, "epochs": args.ws_epochs},
+           "plain": {k: plain[k] for k in ("newton", "krylov", "wall")},
+           "warm": {k: warm[k] for k in ("newton", "krylov", "wall", "kept")},
+           "speedup": plain["wall"] / warm["wall"], "max_rel_difference": max(errs),
+           "training": {"record_seconds": t_rec, "train_seconds": t_train, "loss_first": hist[0],
+                        "loss_last": hist[-1], "samples": len(samples)}}
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -25,6 +25,11 @@ EXPORTED = [
     "cmg_mass_matrix", "cmg_residual_device", "cmg_residual_tangent_device", "cmg_element_kernel_device",
     "cmg_spmv", "cmg_linear_solve", "cmg_set_state", "cmg_get_state", "cmg_set_initial_guess",
     "cmg_static_step", "cmg_newmark_step", "cmg_copy_device",
+    "cmg_batch_create", "cmg_batch_destroy", "cmg_batch_configure", "cmg_batch_info_get", "cmg_batch_static_step",
+    "cmg_batch_get_nodes", "cmg_batch_set_nodes", "cmg_batch_residual", "cmg_batch_cavity_volumes",
+    "cmg_mlp_num_weights", "cmg_mlp_create", "cmg_mlp_destroy", "cmg_mlp_get_weights", "cmg_mlp_set_weights",
+    "cmg_mlp_set_scales", "cmg_mlp_features", "cmg_mlp_targets", "cmg_mlp_forward", "cmg_mlp_loss_grad",
+    "cmg_mlp_adam_step", "cmg_mlp_propose_guess",
 ]
 
 
@@ -65,10 +70,24 @@ class CmgStepStats(ctypes.Structure):
                 ("converged_reason", ctypes.c_int32), ("corrupt_element", ctypes.c_int32),
                 ("residual_norm", ctypes.c_double), ("initial_residual_norm", ctypes.c_double),
                 ("step_norm", ctypes.c_double), ("wall_seconds", ctypes.c_double),
-                ("tangent_seconds", ctypes.c_double), ("krylov_seconds", ctypes.c_double)]
+                ("tangent_seconds", ctypes.c_double), ("krylov_seconds", ctypes.c_double),
+                ("guess_source", ctypes.c_int32), ("reserved_s", ctypes.c_int32),
+                ("guess_residual_ratio", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CmgBatchInfo(ctypes.Structure):
+    _fields_ = [("n_sets", ctypes.c_int32), ("element_variant", ctypes.c_int32), ("spmv_lanes", ctypes.c_int32),
+                ("cg_chunk", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("tuned", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2), ("element_ms", ctypes.c_double * 3),
+                ("spmv_ms", ctypes.c_double * 3), ("tune_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"n_sets": self.n_sets, "element_variant": self.element_variant, "spmv_lanes": self.spmv_lanes,
+                "cg_chunk": self.cg_chunk, "use_graph": self.use_graph, "tuned": self.tuned,
+                "element_ms": list(self.element_ms), "spmv_ms": list(self.spmv_ms)}
 
 
 _lib = None
@@ -114,6 +133,27 @@ def load():
         "cmg_static_step": (ctypes.c_int, [P, dbl, dbl, P]),
         "cmg_newmark_step": (ctypes.c_int, [P, dbl, dbl, P]),
         "cmg_copy_device": (ctypes.c_int, [P, P, P, i64]),
+        "cmg_batch_create": (ctypes.c_int, [P, i32, P, ctypes.POINTER(P)]),
+        "cmg_batch_destroy": (ctypes.c_int, [P]),
+        "cmg_batch_configure": (ctypes.c_int, [P, i32, i32, i32, i32]),
+        "cmg_batch_info_get": (ctypes.c_int, [P, P]),
+        "cmg_batch_static_step": (ctypes.c_int, [P, dbl, dbl, P]),
+        "cmg_batch_get_nodes": (ctypes.c_int, [P, P]),
+        "cmg_batch_set_nodes": (ctypes.c_int, [P, P]),
+        "cmg_batch_residual": (ctypes.c_int, [P, P, dbl, P]),
+        "cmg_batch_cavity_volumes": (ctypes.c_int, [P, P]),
+        "cmg_mlp_num_weights": (i32, [i32]),
+        "cmg_mlp_create": (ctypes.c_int, [P, i32, ctypes.c_uint64, ctypes.POINTER(P)]),
+        "cmg_mlp_destroy": (ctypes.c_int, [P]),
+        "cmg_mlp_get_weights": (ctypes.c_int, [P, P]),
+        "cmg_mlp_set_weights": (ctypes.c_int, [P, P]),
+        "cmg_mlp_set_scales": (ctypes.c_int, [P, dbl, dbl, dbl]),
+        "cmg_mlp_features": (ctypes.c_int, [P, dbl, dbl, P]),
+        "cmg_mlp_targets": (ctypes.c_int, [P, P]),
+        "cmg_mlp_forward": (ctypes.c_int, [P, P, i64, i32, P]),
+        "cmg_mlp_loss_grad": (ctypes.c_int, [P, P, P, i64, P, P]),
+        "cmg_mlp_adam_step": (ctypes.c_int, [P, ctypes.c_float]),
+        "cmg_mlp_propose_guess": (ctypes.c_int, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

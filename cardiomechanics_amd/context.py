@@ -109,9 +109,22 @@ class MechanicsContext:
         return rc
 
     def close(self):
+        # batches and networks made on this context go first
+        for dep in list(getattr(self, "_dependents", [])):
+            obj = dep()
+            if obj is not None:
+                obj.close()
+        self._dependents = []
         if getattr(self, "h", None):
             self.L.cmg_destroy(self.h)
             self.h = None
+
+    def _add_dependent(self, obj):
+        import weakref
+
+        if not hasattr(self, "_dependents"):
+            self._dependents = []
+        self._dependents.append(weakref.ref(obj))
 
     def __del__(self):
         try:
@@ -227,3 +240,78 @@ class MechanicsContext:
         if rc not in (_lib.SUCCESS, _lib.FAILED):
             raise CmgError(rc, "cmg_newmark_step")
         return rc, st
+
+
+class ParameterBatch:
+    """S material-parameter sets on the mesh of one `MechanicsContext` (cmg_batch_* in
+    include/cmg.h): every set runs the quasi-static step of the context, all sets in
+    the same kernel launches.  `material_sets` is a list of S lists of materials, each
+    in the context's material order."""
+
+    def __init__(self, ctx: MechanicsContext, material_sets, use_graph=True, spmv_lanes=0, element_variant=0,
+                 cg_chunk=16):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.S = len(material_sets)
+        flat = [to_cmg_material(m) for ms in material_sets for m in ms]
+        if len(flat) != self.S * len(ctx.materials):
+            raise ValueError("every set needs one material per context material")
+        arr = (_lib.CmgMaterial * len(flat))(*flat)
+        h = ctypes.c_void_p()
+        rc = self.L.cmg_batch_create(ctx.h, self.S, arr, ctypes.byref(h))
+        if rc != _lib.SUCCESS:
+            raise CmgError(rc, "cmg_batch_create")
+        self.h = h
+        ctx._add_dependent(self)
+        ctx._check(self.L.cmg_batch_configure(self.h, int(use_graph), spmv_lanes, element_variant, cg_chunk),
+                   "cmg_batch_configure")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cmg_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        i = _lib.CmgBatchInfo()
+        self.ctx._check(self.L.cmg_batch_info_get(self.h, ctypes.byref(i)), "cmg_batch_info_get")
+        return i.as_dict()
+
+    def static_step(self, time, dt):
+        st = (_lib.CmgStepStats * self.S)()
+        rc = self.L.cmg_batch_static_step(self.h, time, dt, st)
+        if rc not in (_lib.SUCCESS, _lib.FAILED):
+            raise CmgError(rc, "cmg_batch_static_step")
+        return rc, list(st)
+
+    def nodes(self):
+        out = np.zeros((self.S, self.ctx.n_dofs))
+        self.ctx._check(self.L.cmg_batch_get_nodes(self.h, _ptr(out)), "cmg_batch_get_nodes")
+        return out
+
+    def set_nodes(self, nodes):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(nodes, np.float64).reshape(-1, self.ctx.n_dofs),
+                                                 (self.S, self.ctx.n_dofs)))
+        self.ctx._check(self.L.cmg_batch_set_nodes(self.h, _ptr(a)), "cmg_batch_set_nodes")
+
+    def residual(self, x, time=0.0):
+        x = np.ascontiguousarray(np.asarray(x, np.float64).reshape(self.S, self.ctx.n_dofs))
+        f = np.zeros_like(x)
+        self.ctx._check(self.L.cmg_batch_residual(self.h, _ptr(x), time, _ptr(f)), "cmg_batch_residual")
+        return f
+
+    def cavity_volumes(self):
+        v = np.zeros((self.S, self.ctx.n_groups))
+        self.ctx._check(self.L.cmg_batch_cavity_volumes(self.h, _ptr(v)), "cmg_batch_cavity_volumes")
+        return v

@@ -16,9 +16,13 @@ namespace {
 __global__ __launch_bounds__(256) void k_gather_vector(int64_t N, const int64_t* inc_ptr, const int32_t* inc,
                                                        const double* fe, const int64_t* sinc_ptr,
                                                        const int32_t* sinc, const double* fs,
-                                                       const uint8_t* bcmask, double* f) {
+                                                       const uint8_t* bcmask, double* f, SetStrides b) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= N) return;
+    const int64_t set = blockIdx.y;
+    if (r >= N || (b.active && !b.active[set])) return;
+    fe += set * b.a;
+    if (fs) fs += set * b.b;
+    f += set * b.c;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int64_t k = inc_ptr[r]; k < inc_ptr[r + 1]; k++) {
         const double* p = fe + 3 * (int64_t)inc[k];
@@ -64,9 +68,13 @@ __global__ __launch_bounds__(256) void k_gather_matrix(int64_t nnzb, const int64
                                                        const KT* Ke, const double* Ks, int64_t n_solid,
                                                        const int32_t* blkrow, const int32_t* colind,
                                                        const double* mval, double cm, const uint8_t* bcmask,
-                                                       VT* vals) {
+                                                       VT* vals, SetStrides st) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 9 * nnzb) return;
+    const int64_t set = blockIdx.y;
+    if (t >= 9 * nnzb || (st.active && !st.active[set])) return;
+    Ke += set * st.a;
+    if (Ks) Ks += set * st.b;
+    vals += set * st.c;
     const int64_t slot = t / 9;
     const int k = (int)(t - 9 * slot);
     const int i = k / 3, j = k - 3 * (k / 3);
@@ -101,10 +109,10 @@ __global__ __launch_bounds__(256) void k_gather_matrix(int64_t nnzb, const int64
 
 void launch_gather_vector(int64_t N, const int64_t* inc_ptr, const int32_t* inc, const double* fe, int,
                           const int64_t* sinc_ptr, const int32_t* sinc, const double* fs, int,
-                          const uint8_t* bcmask, double* f, hipStream_t s) {
+                          const uint8_t* bcmask, double* f, hipStream_t s, const SetStrides& b) {
     if (N == 0) return;
-    hipLaunchKernelGGL(k_gather_vector, dim3((N + 255) / 256), dim3(256), 0, s, N, inc_ptr, inc, fe, sinc_ptr, sinc,
-                       fs, bcmask, f);
+    hipLaunchKernelGGL(k_gather_vector, dim3((N + 255) / 256, b.n_sets), dim3(256), 0, s, N, inc_ptr, inc, fe,
+                       sinc_ptr, sinc, fs, bcmask, f, b);
 }
 
 void launch_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval, double cm,
@@ -117,24 +125,24 @@ void launch_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colin
 void launch_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* contrib, const void* Ke, bool fp32,
                           const double* Ks, int64_t n_solid_blocks, const int32_t* blkrow, const int32_t* colind,
                           const double* mval, double cm, const uint8_t* bcmask, void* vals, bool vals_fp32,
-                          hipStream_t s) {
+                          hipStream_t s, const SetStrides& b) {
     if (nnzb == 0) return;
-    const dim3 grid((unsigned)((9 * nnzb + 255) / 256));
+    const dim3 grid((unsigned)((9 * nnzb + 255) / 256), b.n_sets);
     if (fp32) {
         if (vals_fp32)
             hipLaunchKernelGGL((k_gather_matrix<float, float>), grid, dim3(256), 0, s, nnzb, cptr, contrib,
-                               (const float*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (float*)vals);
+                               (const float*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (float*)vals, b);
         else
             hipLaunchKernelGGL((k_gather_matrix<float, double>), grid, dim3(256), 0, s, nnzb, cptr, contrib,
-                               (const float*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (double*)vals);
+                               (const float*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (double*)vals, b);
     } else {
         if (vals_fp32)
             hipLaunchKernelGGL((k_gather_matrix<double, float>), grid, dim3(256), 0, s, nnzb, cptr, contrib,
-                               (const double*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (float*)vals);
+                               (const double*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (float*)vals, b);
         else
             hipLaunchKernelGGL((k_gather_matrix<double, double>), grid, dim3(256), 0, s, nnzb, cptr, contrib,
                                (const double*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask,
-                               (double*)vals);
+                               (double*)vals, b);
     }
 }
 

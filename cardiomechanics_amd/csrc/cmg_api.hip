@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "cmg_internal.h"
+#include "cmg_ctx.h"
 
 namespace cmg {
 namespace {
@@ -22,89 +22,11 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 using namespace cmg;
 
-// ---------------------------------------------------------------------------
-// device buffer helper
-// ---------------------------------------------------------------------------
-namespace {
+namespace cmg {
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    bool alloc(size_t b) {
-        release();
-        if (b == 0) return true;
-        if (hipMalloc(&p, b) != hipSuccess) {
-            p = nullptr;
-            return false;
-        }
-        bytes = b;
-        return true;
-    }
-    template <typename T> T* as() const { return static_cast<T*>(p); }
-};
-
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-}  // namespace
-
-struct cmg_ctx {
-    cmg_options opt;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int npe = 10, nqp = 4, nps = 0;
-    int64_t N = 0, E = 0, S = 0, n = 0;   // nodes, elements, surfaces, dofs
-    int n_groups = 0;
-    bool fp32 = false;
-    // host copies needed for maps / exports
-    std::vector<uint8_t> h_bc;
-    std::vector<int64_t> h_rowptr;
-    std::vector<int32_t> h_colind;
-    std::vector<double> h_mval;
-    std::vector<double> h_pressure;
-    int64_t nnzb = 0;
-    int64_t n_solid_blocks = 0;
-    // element data (SoA)
-    DevBuf conn, dNdX, bases, w, matidx, tact, mats, bcmask;
-    DevBuf fe, Ke;
-    // surfaces
-    DevBuf sconn, sgroup, pressure, fs, Ks, vol_partial;
-    // pattern and maps
-    DevBuf rowptr, colind, blkrow, cptr, contrib, mval;
-    DevBuf inc_ptr, inc, sinc_ptr, sinc;
-    // state and work vectors
-    DevBuf Xref, nodes, xeval, d, dpred, vpred, vel, acc, r, delta, dsave, tmpA, tmpB;
-    DevBuf vals, dinv;
-    DevBuf gm_V, gm_partial, gm_h, gm_y;
-    int gm_m = 0;
-    double* h_pinned = nullptr;   // host staging for reductions
-    DevBuf corrupt;
-    // static solver bookkeeping (CBSolverEquilibrium::UpdateInitialGuess)
-    double time_last = 0.0, step_last = 0.0;
-    DevBuf guess;               // normalised previous increment
-    bool has_override = false;  // cmg_set_initial_guess
-    DevBuf override_guess;
-};
-
-namespace {
-
-int fail(const std::string& m, int code = CMG_ERR_INVALID) {
+int fail(const std::string& m, int code) {
     set_error(m);
     return code;
-}
-
-template <typename T>
-int upload(DevBuf& b, const T* src, size_t count, hipStream_t s) {
-    if (!b.alloc(count * sizeof(T))) return fail("hipMalloc failed", CMG_ERR_NOMEM);
-    if (count) CMG_HIP_CHECK(hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
-    return CMG_SUCCESS;
 }
 
 int alloc(DevBuf& b, size_t bytes) {
@@ -112,11 +34,38 @@ int alloc(DevBuf& b, size_t bytes) {
     return CMG_SUCCESS;
 }
 
-#define CMG_TRY(expr)                       \
-    do {                                    \
-        int rc__ = (expr);                  \
-        if (rc__ != CMG_SUCCESS) return rc__; \
-    } while (0)
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int to_dev_material(const cmg_material& m, DevMaterial& d) {
+    if (m.law != CMG_LAW_GUCCIONE && m.law != CMG_LAW_HOLZAPFEL) return fail("unknown material law");
+    std::memset(&d, 0, sizeof(d));
+    d.law = m.law;
+    d.ignore_corrupt = m.ignore_corrupt;
+    d.tension_fn = m.tension_fn;
+    d.density = m.density;
+    for (int k = 0; k < 12; k++) d.p[k] = m.params[k];
+    d.tension_max = m.tension_max;
+    for (int k = 0; k < 12; k++) d.fn[k] = m.fn[k];
+    if (d.tension_fn == CMG_FN_DOUBLEHILL && !(d.fn[11] > 0.0)) {
+        double mx = -INFINITY;   // CBAnalyticFunction.h:155-165
+        for (int q = 0; q <= 1000; q++) {
+            double t = q * d.fn[5] / 1000;
+            double cc = std::pow(t / d.fn[6], d.fn[8]);
+            cc = cc / (1.0 + cc);
+            double rr = std::pow(t / d.fn[7], d.fn[9]);
+            rr = 1.0 / (1.0 + rr);
+            mx = std::max(mx, cc * rr);
+        }
+        d.fn[11] = mx;
+    }
+    return CMG_SUCCESS;
+}
+
+}  // namespace cmg
+
+namespace {
 
 // T4 shape derivatives, CBElementSolidT4::CalcShapeFunctionDerivatives (:73-121)
 double t4_shape(const double* X, double* g) {
@@ -183,6 +132,7 @@ void cmg_default_options(cmg_options* o) {
     o->newton_maxit = 50;
     o->krylov_restart = 80;
     o->krylov_maxit = 4000;
+    o->krylov_method = 1;   // BiCGStab: about half the time of GMRES(80) per Newmark step on the LV
     o->device = 0;
     o->element_variant = 2;
     o->newton_atol = 1e-8;
@@ -259,31 +209,8 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     if ((int64_t)E * npe * npe >= ((int64_t)1 << 60)) return fail("cmg_create: mesh too large");
     // -- materials
     std::vector<DevMaterial> dm(n_materials);
-    for (int i = 0; i < n_materials; i++) {
-        const cmg_material& m = materials[i];
-        if (m.law != CMG_LAW_GUCCIONE && m.law != CMG_LAW_HOLZAPFEL) return fail("cmg_create: unknown law");
-        DevMaterial& d = dm[i];
-        std::memset(&d, 0, sizeof(d));
-        d.law = m.law;
-        d.ignore_corrupt = m.ignore_corrupt;
-        d.tension_fn = m.tension_fn;
-        d.density = m.density;
-        for (int k = 0; k < 12; k++) d.p[k] = m.params[k];
-        d.tension_max = m.tension_max;
-        for (int k = 0; k < 12; k++) d.fn[k] = m.fn[k];
-        if (d.tension_fn == CMG_FN_DOUBLEHILL && !(d.fn[11] > 0.0)) {
-            double mx = -INFINITY;   // CBAnalyticFunction.h:155-165
-            for (int q = 0; q <= 1000; q++) {
-                double t = q * d.fn[5] / 1000;
-                double cc = std::pow(t / d.fn[6], d.fn[8]);
-                cc = cc / (1.0 + cc);
-                double rr = std::pow(t / d.fn[7], d.fn[9]);
-                rr = 1.0 / (1.0 + rr);
-                mx = std::max(mx, cc * rr);
-            }
-            d.fn[11] = mx;
-        }
-    }
+    for (int i = 0; i < n_materials; i++) CMG_TRY(to_dev_material(materials[i], dm[i]));
+    c->n_mats = n_materials;
     CMG_TRY(upload(c->mats, dm.data(), dm.size(), s));
     // -- per-element SoA data
     std::vector<int32_t> conn_soa((size_t)npe * E);
@@ -457,10 +384,10 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     // -- work buffers
     const size_t vb = (size_t)c->n * sizeof(double);
     for (DevBuf* b : {&c->nodes, &c->xeval, &c->d, &c->dpred, &c->vpred, &c->vel, &c->acc, &c->r, &c->delta,
-                      &c->dsave, &c->tmpA, &c->tmpB, &c->guess, &c->override_guess})
+                      &c->dsave, &c->tmpA, &c->tmpB, &c->guess, &c->override_guess, &c->inc1, &c->inc2})
         CMG_TRY(alloc(*b, vb));
     CMG_HIP_CHECK(hipMemcpyAsync(c->nodes.p, mesh->ref_coords, vb, hipMemcpyHostToDevice, s));
-    for (DevBuf* b : {&c->d, &c->vel, &c->acc, &c->guess, &c->dpred, &c->vpred})
+    for (DevBuf* b : {&c->d, &c->vel, &c->acc, &c->guess, &c->dpred, &c->vpred, &c->inc1, &c->inc2})
         launch_fill(c->n, 0.0, b->as<double>(), s);
     CMG_TRY(alloc(c->fe, (size_t)E * 3 * npe * sizeof(double)));
     CMG_TRY(alloc(c->Ke, (size_t)E * 9 * npe * npe * (c->fp32 ? sizeof(float) : sizeof(double))));
@@ -516,7 +443,7 @@ int cmg_set_pressures(cmg_ctx* c, const double* p) {
 // ===========================================================================
 // internal evaluation helpers
 // ===========================================================================
-namespace {
+namespace cmg {
 
 ElemArgs elem_args(cmg_ctx* c, const double* x_dev, double time) {
     ElemArgs a;
@@ -550,6 +477,10 @@ SurfArgs surf_args(cmg_ctx* c, const double* x_dev, bool with_volume) {
     a.vol_partial = with_volume ? c->vol_partial.as<double>() : nullptr;
     return a;
 }
+
+}  // namespace cmg
+
+namespace {
 
 // element kernels (+ surfaces) at x_dev; returns CMG_SUCCESS or CMG_CORRUPT_ELEMENT
 int eval_elements(cmg_ctx* c, const double* x_dev, double time, bool tangent, bool pressure, int* corrupt_id) {
@@ -1044,12 +975,30 @@ int cmg_set_initial_guess(cmg_ctx* c, const double* inc) {
     CMG_HIP_CHECK(hipMemcpyAsync(c->override_guess.p, inc, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
     c->has_override = true;
+    c->override_checked = false;
     return CMG_SUCCESS;
 }
 
 }  // extern "C"
 
 namespace {
+
+// ||r(d)|| of the Newmark residual at the current increment d (infinite when an
+// element inverts)
+int residual_norm_at(cmg_ctx* c, double time, double cm, double* out) {
+    hipStream_t s = c->stream;
+    launch_waxpby(c->n, 1.0, c->nodes.as<double>(), 1.0, c->d.as<double>(), c->xeval.as<double>(), s);
+    int corrupt = -1;
+    if (eval_elements(c, c->xeval.as<double>(), time, false, true, &corrupt) != CMG_SUCCESS) {
+        *out = INFINITY;
+        return CMG_SUCCESS;
+    }
+    gather_residual(c, true, c->r.as<double>());
+    if (cm != 0.0)
+        launch_add_mass_term(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), c->mval.as<double>(), cm,
+                             c->d.as<double>(), c->dpred.as<double>(), c->bcmask.as<uint8_t>(), c->r.as<double>(), s);
+    return norm2(c, c->r.as<double>(), out);
+}
 
 // Newton on r(d) = f_int(nodes + d) + f_p(nodes + d) [+ cm M (d - d~)] = 0, the
 // SNES newtonls loop of CBSolverNewmarkBeta.cpp:890 / CBSolverEquilibrium.cpp:287 with
@@ -1185,6 +1134,7 @@ int cmg_static_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
     if (c->has_override) {
         launch_scale_copy(n, 1.0, c->override_guess.as<double>(), d, s);
         c->has_override = false;
+        st->guess_source = 1;
     }
     launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
     int rc = newton(c, time, 0.0, st);
@@ -1211,14 +1161,29 @@ int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) 
     // predictor (:845-855): the initial guess is d~
     launch_newmark_predict(n, dt, beta, gamma, c->vel.as<double>(), c->acc.as<double>(), c->dpred.as<double>(),
                            c->vpred.as<double>(), s);
-    if (c->has_override) {
-        launch_scale_copy(n, 1.0, c->override_guess.as<double>(), c->d.as<double>(), s);
-        c->has_override = false;
-    } else {
-        launch_scale_copy(n, 1.0, c->dpred.as<double>(), c->d.as<double>(), s);
-    }
-    launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), c->d.as<double>(), s);
     const double cm = 1.0 / (beta * dt * dt);
+    double* d = c->d.as<double>();
+    if (c->has_override && c->override_checked) {
+        // proposed guess (learned warm start): kept only if its residual is below the predictor's
+        double r_pred, r_prop;
+        launch_scale_copy(n, 1.0, c->dpred.as<double>(), d, s);
+        launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
+        CMG_TRY(residual_norm_at(c, time, cm, &r_pred));
+        launch_scale_copy(n, 1.0, c->override_guess.as<double>(), d, s);
+        launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
+        CMG_TRY(residual_norm_at(c, time, cm, &r_prop));
+        st->guess_source = (r_prop < r_pred) ? 2 : 3;
+        if (st->guess_source == 3) launch_scale_copy(n, 1.0, c->dpred.as<double>(), d, s);
+        st->guess_residual_ratio = r_pred > 0.0 ? r_prop / r_pred : 0.0;
+    } else if (c->has_override) {
+        launch_scale_copy(n, 1.0, c->override_guess.as<double>(), d, s);
+        st->guess_source = 1;
+    } else {
+        launch_scale_copy(n, 1.0, c->dpred.as<double>(), d, s);
+    }
+    c->has_override = false;
+    c->override_checked = false;
+    launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
     int rc = newton(c, time, cm, st);
     if (rc != CMG_SUCCESS) {
         CMG_HIP_CHECK(hipStreamSynchronize(s));
@@ -1228,6 +1193,9 @@ int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) 
     launch_newmark_correct(n, dt, beta, gamma, c->d.as<double>(), c->dpred.as<double>(), c->vpred.as<double>(),
                            c->vel.as<double>(), c->acc.as<double>(), s);
     launch_axpby(n, 1.0, c->d.as<double>(), 1.0, c->nodes.as<double>(), s);
+    // increment history (features of a learned warm start)
+    launch_scale_copy(n, 1.0, c->inc1.as<double>(), c->inc2.as<double>(), s);
+    launch_scale_copy(n, 1.0, c->d.as<double>(), c->inc1.as<double>(), s);
     CMG_HIP_CHECK(hipStreamSynchronize(s));
     return CMG_SUCCESS;
 }

@@ -1,0 +1,100 @@
+// Private layout of a mechanics context, shared by the translation units of the
+// library (cmg_api.hip, batch.hip).  Not part of the ABI.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "cmg_internal.h"
+
+namespace cmg {
+
+// owning device allocation
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    bool alloc(size_t b) {
+        release();
+        if (b == 0) return true;
+        if (hipMalloc(&p, b) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        bytes = b;
+        return true;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+int fail(const std::string& m, int code = CMG_ERR_INVALID);
+int to_dev_material(const cmg_material& m, DevMaterial& d);   // validates the law
+int alloc(DevBuf& b, size_t bytes);
+double now_s();
+
+template <typename T>
+int upload(DevBuf& b, const T* src, size_t count, hipStream_t s) {
+    if (!b.alloc(count * sizeof(T))) return fail("hipMalloc failed", CMG_ERR_NOMEM);
+    if (count) CMG_HIP_CHECK(hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return CMG_SUCCESS;
+}
+
+}  // namespace cmg
+
+#define CMG_TRY(expr)                         \
+    do {                                      \
+        int rc__ = (expr);                    \
+        if (rc__ != CMG_SUCCESS) return rc__; \
+    } while (0)
+
+struct cmg_ctx {
+    cmg_options opt;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int npe = 10, nqp = 4, nps = 0;
+    int64_t N = 0, E = 0, S = 0, n = 0;   // nodes, elements, surfaces, dofs
+    int n_groups = 0;
+    int n_mats = 0;
+    bool fp32 = false;
+    // host copies needed for maps / exports
+    std::vector<uint8_t> h_bc;
+    std::vector<int64_t> h_rowptr;
+    std::vector<int32_t> h_colind;
+    std::vector<double> h_mval;
+    std::vector<double> h_pressure;
+    int64_t nnzb = 0;
+    int64_t n_solid_blocks = 0;
+    // element data (SoA)
+    cmg::DevBuf conn, dNdX, bases, w, matidx, tact, mats, bcmask;
+    cmg::DevBuf fe, Ke;
+    // surfaces
+    cmg::DevBuf sconn, sgroup, pressure, fs, Ks, vol_partial;
+    // pattern and maps
+    cmg::DevBuf rowptr, colind, blkrow, cptr, contrib, mval;
+    cmg::DevBuf inc_ptr, inc, sinc_ptr, sinc;
+    // state and work vectors
+    cmg::DevBuf Xref, nodes, xeval, d, dpred, vpred, vel, acc, r, delta, dsave, tmpA, tmpB;
+    cmg::DevBuf vals, dinv;
+    cmg::DevBuf gm_V, gm_partial, gm_h, gm_y;
+    int gm_m = 0;
+    double* h_pinned = nullptr;   // host staging for reductions
+    cmg::DevBuf corrupt;
+    // static solver bookkeeping (CBSolverEquilibrium::UpdateInitialGuess)
+    double time_last = 0.0, step_last = 0.0;
+    cmg::DevBuf guess;               // normalised previous increment
+    bool has_override = false;  // cmg_set_initial_guess / cmg_mlp_propose_guess
+    bool override_checked = false;   // compare with the predictor before use (Newmark)
+    cmg::DevBuf inc1, inc2;     // increments of the last two Newmark steps
+    cmg::DevBuf override_guess;
+};
+
+namespace cmg {
+// kernel argument packs of a context for coordinates x_dev (device pointer)
+ElemArgs elem_args(cmg_ctx* c, const double* x_dev, double time);
+SurfArgs surf_args(cmg_ctx* c, const double* x_dev, bool with_volume);
+}  // namespace cmg

@@ -30,32 +30,43 @@ constexpr int kResidualBlock = 128;
 
 // Per-element data, structure-of-arrays: entry k of element e at [k * E + e].
 struct ElemArgs {
-    int64_t E;
-    const int32_t* conn;     // [npe][E]
-    const double* dNdX;      // [nqp*npe*3][E]  reference dN/dX at the quadrature points
-    const double* bases;     // [nqp*9][E]      fibre frame (columns f, s, n) per quadrature point
-    const double* w;         // [E]  detJ/24 (T10) or volume (T4)
-    const int32_t* matidx;   // [E] or nullptr
-    const double* tact;      // [E]
-    const DevMaterial* mats;
-    const double* x;         // current coordinates [3N]
-    double time;
-    double* fe;              // [E][3npe] element forces
-    void* Ke;                // [E][npe][npe][3][3] element tangents (double or float)
-    int* corrupt;            // smallest element id with det F <= 0 (INT_MAX if none)
+    int64_t E = 0;
+    const int32_t* conn = nullptr;     // [npe][E]
+    const double* dNdX = nullptr;      // [nqp*npe*3][E]  reference dN/dX at the quadrature points
+    const double* bases = nullptr;     // [nqp*9][E]      fibre frame (columns f, s, n) per quadrature point
+    const double* w = nullptr;         // [E]  detJ/24 (T10) or volume (T4)
+    const int32_t* matidx = nullptr;   // [E] or nullptr
+    const double* tact = nullptr;      // [E]
+    const DevMaterial* mats = nullptr;
+    const double* x = nullptr;         // current coordinates [3N]
+    double time = 0.0;
+    double* fe = nullptr;              // [E][3npe] element forces
+    void* Ke = nullptr;                // [E][npe][npe][3][3] element tangents (double or float)
+    int* corrupt = nullptr;            // smallest element id with det F <= 0 (INT_MAX if none)
+    // Parameter-set batch: blockIdx.y selects the set.  Set s reads x + s*set_x and
+    // materials mats + s*set_mats, writes fe + s*set_fe, Ke + s*set_Ke_bytes and
+    // corrupt[s]; sets whose active flag is 0 are skipped.
+    int n_sets = 1;
+    int set_mats = 0;
+    int64_t set_x = 0, set_fe = 0, set_Ke_bytes = 0;
+    const int32_t* active = nullptr;
 };
 
 struct SurfArgs {
-    int64_t S;
-    int nps;
-    const int32_t* conn;     // [S][nps]
-    const int32_t* group;    // [S]
-    const double* pressure;  // [n_groups]
-    const uint8_t* bcmask;   // [N]
-    const double* x;         // [3N]
-    double* fs;              // [S][3nps]
-    double* Ks;              // [S][3nps][3nps] or nullptr
-    double* vol_partial;     // [S] or nullptr
+    int64_t S = 0;
+    int nps = 3;
+    const int32_t* conn = nullptr;     // [S][nps]
+    const int32_t* group = nullptr;    // [S]
+    const double* pressure = nullptr;  // [n_groups]
+    const uint8_t* bcmask = nullptr;   // [N]
+    const double* x = nullptr;         // [3N]
+    double* fs = nullptr;              // [S][3nps]
+    double* Ks = nullptr;              // [S][3nps][3nps] or nullptr
+    double* vol_partial = nullptr;     // [S] or nullptr
+    // parameter-set batch (blockIdx.y = set), as in ElemArgs
+    int n_sets = 1;
+    int64_t set_x = 0, set_fs = 0, set_Ks = 0, set_vol = 0;
+    const int32_t* active = nullptr;
 };
 
 // ---- launchers (element_kernels.hip) ----
@@ -67,9 +78,15 @@ void launch_t10_shape(int64_t E, const int32_t* conn_aos, const double* X, doubl
 
 // ---- launchers (assembly_kernels.hip) ----
 // vector gather: f[dof] = sum over incidences; BC components zeroed
+// Batched form (n_sets > 1): set k reads fe + k*set_fe, fs + k*set_fs and writes f + k*set_f.
+struct SetStrides {
+    int n_sets = 1;
+    int64_t a = 0, b = 0, c = 0;   // per-set strides of the kernel's arrays, in elements
+    const int32_t* active = nullptr;
+};
 void launch_gather_vector(int64_t N, const int64_t* inc_ptr, const int32_t* inc, const double* fe, int stride_e,
                           const int64_t* sinc_ptr, const int32_t* sinc, const double* fs, int stride_s,
-                          const uint8_t* bcmask, double* f, hipStream_t s);
+                          const uint8_t* bcmask, double* f, hipStream_t s, const SetStrides& b = SetStrides());
 // f += cm * M (d - dt) on free DOFs (Newmark inertia term); M stored as one scalar per block
 void launch_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval, double cm,
                           const double* d, const double* dpred, const uint8_t* bcmask, double* f, hipStream_t s);
@@ -77,7 +94,7 @@ void launch_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colin
 void launch_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* contrib, const void* Ke, bool fp32,
                           const double* Ks, int64_t n_solid_blocks, const int32_t* blkrow, const int32_t* colind,
                           const double* mval, double cm, const uint8_t* bcmask, void* vals, bool vals_fp32,
-                          hipStream_t s);
+                          hipStream_t s, const SetStrides& b = SetStrides());   // strides: Ke, Ks, vals
 
 // ---- launchers (linalg_kernels.hip) ----
 void launch_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals, bool fp32,
@@ -85,7 +102,8 @@ void launch_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind, const 
 void launch_spmv_scalar_mass(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval,
                              const uint8_t* bcmask, const double* x, double* y, hipStream_t s);
 void launch_block_jacobi_setup(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals,
-                               bool fp32, double* dinv, hipStream_t s);
+                               bool fp32, double* dinv, hipStream_t s,
+                               const SetStrides& b = SetStrides());   // strides: vals, dinv
 void launch_block_jacobi_apply(int64_t N, const double* dinv, const double* r, double* z, hipStream_t s);
 // partial dot products of k vectors V[j] (stride ld) with w: out[block][j]; returns #blocks
 int dot_blocks(int64_t n);

@@ -310,11 +310,39 @@ __device__ __forceinline__ void local_F(const double* x, const double (*g)[3], c
 }
 
 // ---------------------------------------------------------------------------
+// parameter-set batch: blockIdx.y selects the set (see ElemArgs)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool batch_select(const ElemArgs& a0, ElemArgs& a) {
+    const int64_t set = blockIdx.y;
+    if (a0.active && !a0.active[set]) return false;
+    a = a0;
+    a.x += set * a0.set_x;
+    a.fe += set * a0.set_fe;
+    a.Ke = static_cast<char*>(a0.Ke) + set * a0.set_Ke_bytes;
+    a.mats += set * a0.set_mats;
+    a.corrupt += set;
+    return true;
+}
+
+__device__ __forceinline__ bool batch_select(const SurfArgs& a0, SurfArgs& a) {
+    const int64_t set = blockIdx.y;
+    if (a0.active && !a0.active[set]) return false;
+    a = a0;
+    a.x += set * a0.set_x;
+    a.fs += set * a0.set_fs;
+    if (a.Ks) a.Ks += set * a0.set_Ks;
+    if (a.vol_partial) a.vol_partial += set * a0.set_vol;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // residual only: one thread per element, FP64
 // ---------------------------------------------------------------------------
 template <int NPE, int NQP>
-__global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a) {
+__global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a0) {
 #pragma clang fp contract(off)
+    ElemArgs a;
+    if (!batch_select(a0, a)) return;
     const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int64_t e = (int64_t)bid * kResidualBlock + threadIdx.x;
     if (e >= a.E) return;
@@ -369,7 +397,9 @@ __global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a) {
 // residual + tangent: 9 lanes per element (see file header)
 // ---------------------------------------------------------------------------
 template <int NPE, int NQP, typename TT>
-__global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a) {
+__global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
+    ElemArgs a;
+    if (!batch_select(a0, a)) return;
     const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x;
     const int slot = lane / 9;
@@ -466,8 +496,10 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a) {
 constexpr int kQRec = 51;   // Ft(9) BBt(9) S(9) Z(9) Y|X(9) czz codot ceq d00 d11 mfs
 
 template <int NPE, int NQP, typename TT, int NSPLIT>
-__global__ __launch_bounds__(64) void k_residual_tangent_v2(const ElemArgs a) {
+__global__ __launch_bounds__(64) void k_residual_tangent_v2(const ElemArgs a0) {
     constexpr int EPW = kTanElemsPerWave;
+    ElemArgs a;
+    if (!batch_select(a0, a)) return;   // uniform per workgroup: no barrier below is skipped by part of it
     __shared__ double sT[EPW][NQP][3 * NPE];
     __shared__ TT sG[EPW][NQP][NPE * 3];
     __shared__ TT sQ[EPW][NQP][kQRec];
@@ -748,7 +780,9 @@ __device__ void t6_forces(const T* x, double p, T* f) {
 }
 
 template <int NPS>
-__global__ __launch_bounds__(64) void k_surface(const SurfArgs a, bool tangent) {
+__global__ __launch_bounds__(64) void k_surface(const SurfArgs a0, bool tangent) {
+    SurfArgs a;
+    if (!batch_select(a0, a)) return;
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.S) return;
     const int32_t* cn = a.conn + s * NPS;
@@ -889,15 +923,15 @@ void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s) {
     if (a.E == 0) return;
     const int nblk = (int)((a.E + kResidualBlock - 1) / kResidualBlock);
     if (npe == 10)
-        hipLaunchKernelGGL((k_residual<10, 4>), dim3(nblk), dim3(kResidualBlock), 0, s, a);
+        hipLaunchKernelGGL((k_residual<10, 4>), dim3(nblk, a.n_sets), dim3(kResidualBlock), 0, s, a);
     else
-        hipLaunchKernelGGL((k_residual<4, 1>), dim3(nblk), dim3(kResidualBlock), 0, s, a);
+        hipLaunchKernelGGL((k_residual<4, 1>), dim3(nblk, a.n_sets), dim3(kResidualBlock), 0, s, a);
 }
 
 void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s, int variant) {
     if (a.E == 0) return;
     const int nblk = (int)((a.E + kTanElemsPerWave - 1) / kTanElemsPerWave);
-#define CMG_LAUNCH(K) hipLaunchKernelGGL(K, dim3(nblk), dim3(64), 0, s, a)
+#define CMG_LAUNCH(K) hipLaunchKernelGGL(K, dim3(nblk, a.n_sets), dim3(64), 0, s, a)
     if (npe == 10) {
         if (variant == 1) {
             if (fp32) CMG_LAUNCH((k_residual_tangent<10, 4, float>));
@@ -925,9 +959,9 @@ void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s) {
     if (a.S == 0) return;
     const int nblk = (int)((a.S + 63) / 64);
     if (a.nps == 3)
-        hipLaunchKernelGGL((k_surface<3>), dim3(nblk), dim3(64), 0, s, a, tangent);
+        hipLaunchKernelGGL((k_surface<3>), dim3(nblk, a.n_sets), dim3(64), 0, s, a, tangent);
     else
-        hipLaunchKernelGGL((k_surface<6>), dim3(nblk), dim3(64), 0, s, a, tangent);
+        hipLaunchKernelGGL((k_surface<6>), dim3(nblk, a.n_sets), dim3(64), 0, s, a, tangent);
 }
 
 void launch_t10_shape(int64_t E, const int32_t* conn_aos, const double* X, double* dNdX_soa, double* w,

@@ -61,9 +61,12 @@ __global__ __launch_bounds__(256) void k_spmv_scalar_mass(int64_t N, const int64
 
 template <typename VT>
 __global__ __launch_bounds__(256) void k_bj_setup(int64_t N, const int64_t* rowptr, const int32_t* colind,
-                                                  const VT* vals, double* dinv) {
+                                                  const VT* vals, double* dinv, SetStrides b) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= N) return;
+    const int64_t set = blockIdx.y;
+    if (r >= N || (b.active && !b.active[set])) return;
+    vals += set * b.a;
+    dinv += set * b.b;
     double a[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
         if (colind[k] == r) {
@@ -244,14 +247,15 @@ void launch_spmv_scalar_mass(int64_t N, const int64_t* rowptr, const int32_t* co
 }
 
 void launch_block_jacobi_setup(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals, bool fp32,
-                               double* dinv, hipStream_t s) {
+                               double* dinv, hipStream_t s, const SetStrides& b) {
     if (N == 0) return;
+    const dim3 grid((unsigned)((N + 255) / 256), b.n_sets);
     if (fp32)
-        hipLaunchKernelGGL((k_bj_setup<float>), grid1(N), dim3(256), 0, s, N, rowptr, colind, (const float*)vals,
-                           dinv);
+        hipLaunchKernelGGL((k_bj_setup<float>), grid, dim3(256), 0, s, N, rowptr, colind, (const float*)vals,
+                           dinv, b);
     else
-        hipLaunchKernelGGL((k_bj_setup<double>), grid1(N), dim3(256), 0, s, N, rowptr, colind, (const double*)vals,
-                           dinv);
+        hipLaunchKernelGGL((k_bj_setup<double>), grid, dim3(256), 0, s, N, rowptr, colind, (const double*)vals,
+                           dinv, b);
 }
 
 void launch_block_jacobi_apply(int64_t N, const double* dinv, const double* r, double* z, hipStream_t s) {

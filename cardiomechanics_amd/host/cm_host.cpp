@@ -566,6 +566,12 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
     o.krylov_rtol = p.Get<double>("Solver.GPU.KrylovTolerance", o.krylov_rtol);
     o.krylov_restart = p.Get<int>("Solver.GPU.KrylovRestart", o.krylov_restart);
     o.krylov_maxit = p.Get<int>("Solver.GPU.KrylovMaxIterations", o.krylov_maxit);
+    {
+        const std::string km = p.Get<std::string>("Solver.GPU.KrylovMethod", "BiCGStab");
+        if (km == "GMRES") o.krylov_method = 0;
+        else if (km == "BiCGStab") o.krylov_method = 1;
+        else throw std::runtime_error("Solver.GPU.KrylovMethod must be GMRES or BiCGStab, not " + km);
+    }
     o.transpose_tangent = p.Get<bool>("Solver.GPU.TransposeTangent", false) ? 1 : 0;
     o.pressure_tangent = p.Get<bool>("Solver.GPU.PressureTangent", true) ? 1 : 0;
     o.device = p.Get<int>("Solver.GPU.Device", 0);

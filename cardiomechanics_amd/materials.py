@@ -153,3 +153,22 @@ def lv_holzapfel(tension_max=60e3, twitch=0.3) -> Material:
     m.tension_max = tension_max
     m.tension_fn, m.fn = function_params("Sinus", start=0.0, stop=twitch, omega=2 * math.pi / twitch, phi=0.0)
     return m
+
+
+def guccione_sweep(n_sets=64, seed=0, K=1e5, density=1082.0):
+    """Passive-parameter sweep sets: C log-uniform in [0.2, 2] kPa, bf U[4, 16], bt U[2, 8],
+    bfs U[4, 16], drawn from numpy.random.default_rng(seed) in that order (one array of
+    n_sets per parameter); penalty K = 1e5 Pa as the reference's LV inflation problem
+    (examples/benchmark2015/Problem2.xml:38).  With K = 1e6 Pa the softest sets
+    (C ~ 0.2 kPa, K/C ~ 5000) stall in damped Newton already at 100 Pa: the Newton
+    direction is isochoric only to first order and the penalty of its second-order
+    volume change dominates the residual along it."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    C = np.exp(rng.uniform(np.log(200.0), np.log(2000.0), n_sets))
+    bf = rng.uniform(4.0, 16.0, n_sets)
+    bt = rng.uniform(2.0, 8.0, n_sets)
+    bfs = rng.uniform(4.0, 16.0, n_sets)
+    return [guccione(float(C[i]), float(bf[i]), float(bt[i]), float(bfs[i]), K, density=density)
+            for i in range(n_sets)]

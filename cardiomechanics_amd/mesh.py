@@ -279,9 +279,11 @@ def lv_mesh(n_target: int = 200_000, dims=None, order: int = 10, surface_type: i
     the base lies in z = 0; the cavity volume taken about the origin
     (CBCirculationCavity.cpp:77-94) is then the volume closed by the base plane.
     Hexes are split into 6 Kuhn tetrahedra; the apex layer uses prisms split in 3.
-    Base nodes are clamped (bc=7).  Surface 1 = endocardium, oriented with its
-    normal out of the cavity (the reference convention: negative pressure
-    amplitude inflates, volume is positive).  Fibres: helix angle linear in t from
+    Base nodes are clamped (bc=7).  Surface 1 = endocardium, oriented like the
+    reference's ellipsoid.sur: normal out of the wall, into the cavity.  The
+    residual carries the pressure forces of CBElementSurfaceT3::CalcForcesDueToPressure
+    (-p A N n), so a negative pressure amplitude inflates (as in the reference's
+    benchmark Problem2.xml) and the cavity volume sum_i x_0.(a x b)/6 is negative.  Fibres: helix angle linear in t from
     fibre_deg[0] (endocardium) to fibre_deg[1] (epicardium), sheet = transmural.
     Activation rises linearly apex -> base over `activation` seconds.
     """
@@ -324,7 +326,7 @@ def lv_mesh(n_target: int = 200_000, dims=None, order: int = 10, surface_type: i
     tets = orient_t4(coords, tets)
     faces = orient_faces_outward(coords, boundary_faces(tets), tets)
     is_endo = np.all(tcoord_v[faces] == 0.0, axis=1)
-    endo = faces[is_endo][:, [0, 2, 1]]                        # flip: normal out of the cavity
+    endo = faces[is_endo]                                      # normal out of the wall, into the cavity
     nvtx = coords.shape[0]
     if order == 10:
         coords, el, edges = t4_to_t10(coords, tets)

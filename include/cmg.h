@@ -108,7 +108,7 @@ typedef struct cmg_options {
     int32_t device;             /* HIP device ordinal (default 0) */
     int32_t element_variant;    /* residual+tangent kernel: 2 shared quadrature stage (default),
                                    3 the same in two node passes, 1 per-lane quadrature points */
-    int32_t krylov_method;      /* 0 GMRES(restart) (default), 1 BiCGStab; both right block-Jacobi */
+    int32_t krylov_method;      /* 0 GMRES(restart), 1 BiCGStab (default); both right block-Jacobi */
     double newton_atol;         /* Solver.Precision used as SNES abstol/rtol/stol (default 1e-8) */
     double newton_rtol;
     double newton_stol;
@@ -137,6 +137,11 @@ typedef struct cmg_step_stats {
     double wall_seconds;
     double tangent_seconds;
     double krylov_seconds;
+    int32_t guess_source;        /* Newton initial guess: 0 predictor / rescaled previous increment,
+                                    1 cmg_set_initial_guess, 2 proposed guess kept (its residual was
+                                    below the predictor's), 3 proposed guess rejected */
+    int32_t reserved_s;
+    double guess_residual_ratio; /* ||r(proposed)|| / ||r(predictor)|| when a guess was proposed */
 } cmg_step_stats;
 
 void cmg_default_options(cmg_options* opt);
@@ -198,8 +203,9 @@ int cmg_copy_device(cmg_ctx* ctx, const double* src_dev, double* dst_dev, int64_
 
 /* ---- linear algebra on the context's pattern ---------------------------- */
 int cmg_spmv(cmg_ctx* ctx, const double* values, const double* x, double* y);
-/* GMRES(m), right preconditioned with 3x3 nodal block Jacobi; replaces the
- * KSP/PC of CBSolverNewmarkBeta.cpp:166-196 (a Krylov solve instead of LU). */
+/* BiCGStab or GMRES(m) (cmg_options.krylov_method), right preconditioned with 3x3
+ * nodal block Jacobi; replaces the KSP/PC of CBSolverNewmarkBeta.cpp:166-196 (a
+ * Krylov solve instead of LU). */
 int cmg_linear_solve(cmg_ctx* ctx, const double* values, const double* b, double* x, double rtol,
                      cmg_solve_stats* stats);
 
@@ -217,6 +223,91 @@ int cmg_static_step(cmg_ctx* ctx, double time, double dt, cmg_step_stats* stats)
 /* CBSolverNewmarkBeta::SolverStep (CBSolverNewmarkBeta.cpp:829-1019): predictor,
  * Newton on M(d - d~)/(beta dt^2) + f_int + f_p = 0, corrector. */
 int cmg_newmark_step(cmg_ctx* ctx, double time, double dt, cmg_step_stats* stats);
+
+/* ---- Parameter-set batch ----------------------------------------------------
+ * No counterpart in the reference, which solves one parameter set per run: S
+ * copies of a context's quasi-static problem with their own material tables, one
+ * mesh.  Each set follows cmg_static_step (CBSolverEquilibrium.cpp:234-363);
+ * every kernel launch covers all sets, and the linear solve is a block-Jacobi
+ * preconditioned conjugate gradient whose scalars stay on the device (runs of
+ * iterations are replayed as one HIP graph).  CG needs symmetric matrices: the
+ * assembled follower-pressure tangent of a closed T3 cavity whose rim is clamped
+ * (the LV endocardium) is symmetric and is included when the context's
+ * pressure_tangent option is set; for open or T6 pressure surfaces create the
+ * context with pressure_tangent = 0 (the reference's T3 tangent is zero,
+ * CBElementSurfaceT3.cpp:331-372). */
+typedef struct cmg_batch cmg_batch;
+
+typedef struct cmg_batch_info {
+    int32_t n_sets;
+    int32_t element_variant;   /* chosen by the autotuner (or fixed by cmg_batch_configure) */
+    int32_t spmv_lanes;        /* lanes per block row of the batched SpMV */
+    int32_t cg_chunk;          /* CG iterations per graph replay */
+    int32_t use_graph;
+    int32_t tuned;             /* 1 once the autotuner has run */
+    int32_t reserved[2];
+    double element_ms[3];      /* measured launch time of element variants 1..3 (0: not timed) */
+    double spmv_ms[3];         /* measured batched SpMV time at 4, 8, 16 lanes */
+    double tune_seconds;
+} cmg_batch_info;
+
+/* materials: [n_sets][n_materials of ctx], same order as at cmg_create.  The batch
+ * shares ctx's mesh, pattern and pressures (cmg_set_pressures on ctx); ctx must
+ * outlive it.  Every set starts from ctx's current node coordinates. */
+int cmg_batch_create(cmg_ctx* ctx, int32_t n_sets, const cmg_material* materials, cmg_batch** out);
+int cmg_batch_destroy(cmg_batch* batch);
+/* use_graph 0/1; spmv_lanes 0 (autotune) or 4/8/16; element_variant 0 (autotune) or 1..3;
+ * cg_chunk >= 1.  Call before the first step. */
+int cmg_batch_configure(cmg_batch* batch, int32_t use_graph, int32_t spmv_lanes, int32_t element_variant,
+                        int32_t cg_chunk);
+int cmg_batch_info_get(cmg_batch* batch, cmg_batch_info* info);
+/* static load step of every set; stats: [n_sets] (may be NULL).  Returns CMG_SUCCESS
+ * if every set converged, CMG_FAILED otherwise (failed sets keep their nodes, as
+ * cmg_static_step). */
+int cmg_batch_static_step(cmg_batch* batch, double time, double dt, cmg_step_stats* stats);
+/* node coordinates of every set [n_sets][3N] (host) */
+int cmg_batch_get_nodes(cmg_batch* batch, double* nodes);
+int cmg_batch_set_nodes(cmg_batch* batch, const double* nodes);
+/* assembled residual (internal + pressure forces) of every set at its own
+ * coordinates x [n_sets][3N] (host) -> f [n_sets][3N] (host) */
+int cmg_batch_residual(cmg_batch* batch, const double* x, double time, double* f);
+/* cavity volumes of every set at its current nodes [n_sets][n_groups] */
+int cmg_batch_cavity_volumes(cmg_batch* batch, double* volumes);
+
+/* ---- Learned Newton warm start -----------------------------------------------
+ * No counterpart in the reference (its Newmark Newton starts from the predictor
+ * d~, CBSolverNewmarkBeta.cpp:845-855).  A per-node MLP 8 -> 128 -> 128 -> 3 (tanh)
+ * maps the features of the next Newmark step of a context
+ *   x = (s_inc * increment of the last step [3], s_inc * increment of the step
+ *        before [3], s_p * pressure rate, s_t * (time - mean activation time of
+ *        the node's elements))
+ * to s_inc * that step's increment.  Trained on the context's own trajectories by
+ * the library's kernels (FP32, mean squared error, Adam).  All float arrays of
+ * these entry points are DEVICE pointers, row-major [rows][8] / [rows][3]. */
+typedef struct cmg_mlp cmg_mlp;
+int32_t cmg_mlp_num_weights(int32_t hidden);   /* -1 unless hidden == 128 */
+/* Glorot-uniform weights from a seeded generator; runs on ctx's device and stream */
+int cmg_mlp_create(cmg_ctx* ctx, int32_t hidden, uint64_t seed, cmg_mlp** out);
+int cmg_mlp_destroy(cmg_mlp* mlp);
+/* flat FP32 weights W1[8][128] b1[128] W2[128][128] b2[128] W3[128][3] b3[3] (host arrays) */
+int cmg_mlp_get_weights(cmg_mlp* mlp, float* weights);
+int cmg_mlp_set_weights(cmg_mlp* mlp, const float* weights);   /* also resets the Adam state */
+int cmg_mlp_set_scales(cmg_mlp* mlp, double inc_scale, double pressure_rate_scale, double phase_scale);
+/* features of the next step at `time` (before cmg_newmark_step) -> X [N][8] */
+int cmg_mlp_features(cmg_mlp* mlp, double time, double pressure_rate, float* X);
+/* scaled increment of the last accepted Newmark step -> T [N][3] */
+int cmg_mlp_targets(cmg_mlp* mlp, float* T);
+/* Y = net(X); bf16 = 1 rounds weights and activations to bfloat16 (FP32 accumulation) */
+int cmg_mlp_forward(cmg_mlp* mlp, const float* X, int64_t rows, int32_t bf16, float* Y);
+/* mean squared error of net(X) against T and its gradient (kept for cmg_mlp_adam_step;
+ * copied to grad [num_weights] on the host when grad is not NULL) */
+int cmg_mlp_loss_grad(cmg_mlp* mlp, const float* X, const float* T, int64_t rows, double* loss, float* grad);
+/* one Adam update (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient */
+int cmg_mlp_adam_step(cmg_mlp* mlp, float lr);
+/* propose Y [N][3] (unscaled by 1/s_inc, constrained DOFs zeroed) as the Newton
+ * initial guess of the next cmg_newmark_step, which keeps it only when its residual
+ * norm is below the predictor's (cmg_step_stats.guess_source 2 or 3) */
+int cmg_mlp_propose_guess(cmg_mlp* mlp, const float* Y);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,8 @@ def test_struct_layouts_match_header_sizes():
     assert ctypes.sizeof(_lib.CmgMaterial) == 16 + 8 * (1 + 12 + 1 + 12)
     assert ctypes.sizeof(_lib.CmgMesh) == 3 * 8 + 4 * 4 + 8 * 8
     assert ctypes.sizeof(_lib.CmgOptions) == 10 * 4 + 7 * 8 + 4 * 8
-    assert ctypes.sizeof(_lib.CmgStepStats) == 4 * 4 + 6 * 8
+    assert ctypes.sizeof(_lib.CmgStepStats) == 4 * 4 + 6 * 8 + 2 * 4 + 8
+    assert ctypes.sizeof(_lib.CmgBatchInfo) == 8 * 4 + 7 * 8
 
 
 def test_default_options_are_the_reference_defaults():
@@ -46,3 +47,30 @@ def test_default_options_are_the_reference_defaults():
     assert o.newton_atol == o.newton_rtol == o.newton_stol == 1e-8   # Solver.Precision
     assert o.newmark_beta == 0.25 and o.newmark_gamma == 0.5          # CBSolverNewmarkBeta.cpp:206-207
     assert o.consistent_mass == 1 and o.fd_epsilon == 1e-12
+
+
+def test_struct_sizes_agree_with_the_c_compiler(tmp_path):
+    """sizeof / offsetof of the header's structs as gcc sees them == the ctypes binding"""
+    import shutil
+    import subprocess
+
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    structs = {"cmg_material": _lib.CmgMaterial, "cmg_mesh": _lib.CmgMesh, "cmg_options": _lib.CmgOptions,
+               "cmg_solve_stats": _lib.CmgSolveStats, "cmg_step_stats": _lib.CmgStepStats,
+               "cmg_batch_info": _lib.CmgBatchInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cmg.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'    printf("{name} %zu\\n", sizeof({name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'    printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sizes"
+    subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().splitlines())
+    for name, cls in structs.items():
+        assert int(got[name]) == ctypes.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert int(got[f"{name}.{f}"]) == getattr(cls, f).offset, f"{name}.{f}"

@@ -336,3 +336,36 @@ def test_t6_flat_triangle_total_force_and_fd_consistency():
     Kc = O.surface_cs_tangent(xd, np.zeros(18, np.uint8), p)
     np.testing.assert_allclose(Kfd, Kc, rtol=0, atol=1e-5 * np.abs(Kc).max())
     assert np.isfinite(K).all()
+
+
+def test_mlp_oracle_against_torch_autograd_and_finite_differences():
+    """The warm-start network's CPU restatement: forward and gradient against torch
+    (an independent implementation, FP64 autograd) and central differences."""
+    import torch
+
+    from oracle import mlp
+
+    H = 16
+    rng = np.random.default_rng(11)
+    w = rng.normal(0, 0.3, mlp.n_weights(H))
+    X = rng.normal(size=(37, 8))
+    T = rng.normal(size=(37, 3))
+    W1, b1, W2, b2, W3, b3 = mlp.unpack(w, H)
+    tw = [torch.tensor(a, dtype=torch.float64, requires_grad=True) for a in (W1, b1, W2, b2, W3, b3)]
+    tx, tt = torch.tensor(X), torch.tensor(T)
+    l1 = torch.nn.functional.linear(tx, tw[0].T, tw[1]).tanh()
+    l2 = torch.nn.functional.linear(l1, tw[2].T, tw[3]).tanh()
+    y = torch.nn.functional.linear(l2, tw[4].T, tw[5])
+    lt = torch.nn.functional.mse_loss(y, tt)
+    lt.backward()
+    gt = np.concatenate([t.grad.numpy().ravel() for t in tw])
+    lo, go = mlp.loss_grad(w, X, T, H)
+    assert abs(lo - lt.item()) <= 1e-14 * abs(lo)
+    assert np.abs(go - gt).max() <= 1e-12 * np.abs(gt).max()
+    # central differences on a few coordinates of every block
+    h = 1e-6
+    for k in [0, 5, 8 * H + 3, 9 * H + 17, 9 * H + H * H + 2, mlp.n_weights(H) - 1]:
+        e = np.zeros_like(w)
+        e[k] = h
+        fd = (mlp.loss(w + e, X, T, H) - mlp.loss(w - e, X, T, H)) / (2 * h)
+        assert abs(fd - go[k]) <= 1e-7 * max(1.0, abs(go[k]))
