@@ -25,6 +25,9 @@ import numpy as np  # noqa: E402
 # f 240 + K_e 7200 (FP64) / 3600 (FP32 values)
 BYTES_PER_ELEM = {0: 1548 + 240 + 7200, 1: 1548 + 240 + 3600}
 FLOPS_PER_ELEM = 42_700
+# measured HBM bytes per element of the FP64 kernel: rocprofv3 FETCH_SIZE + WRITE_SIZE per
+# dispatch, calibrated on a copy kernel of known size (profiles/element_residual_tangent_lv197k_fp64.md)
+TRAFFIC_PER_ELEM = {0: 9_102, 1: None}
 HBM_PEAK = 8.0e12
 
 
@@ -154,8 +157,9 @@ def main():
 
     bytes_launch = BYTES_PER_ELEM[prec] * E
     achieved = bytes_launch / (kernel_ms * 1e-3)
+    traffic = TRAFFIC_PER_ELEM[prec] * E if TRAFFIC_PER_ELEM[prec] else None
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None,
+                "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "bytes_per_element": BYTES_PER_ELEM[prec], "kernel": f"k_residual_tangent variant {best}",
                 "fp64_flops_per_s": FLOPS_PER_ELEM * E / (kernel_ms * 1e-3)}
 
@@ -231,7 +235,9 @@ def sweep(args, prec):
 
     mesh = M.lv_mesh(args.sweep_elements)
     sets = MT.guccione_sweep(args.sweep_sets, seed=0)
-    opts = dict(precision=prec, pressure_tangent=0, krylov_maxit=20000)
+    # the endocardium is a closed T3 cavity with a clamped rim: its assembled pressure tangent is
+    # symmetric, so the batched conjugate gradient keeps it (quadratic Newton convergence)
+    opts = dict(precision=prec, krylov_maxit=20000, newton_atol=1e-12, newton_rtol=1e-10, newton_stol=1e-14)
     steps = args.sweep_steps
     log(f"[sweep] {mesh.n_elements} T10 elements, {mesh.n_nodes} nodes, {len(sets)} Guccione sets, {steps} steps")
     with MechanicsContext(mesh, [sets[0]], **opts) as ctx:
@@ -334,7 +340,9 @@ def warm_start(args, prec):
     t0 = time.perf_counter()
     for b in range(args.ws_train):
         m = MT.lv_holzapfel(tension_max=60e3 * rng.uniform(0.85, 1.15))
-        r = window(m, mesh.activation + rng.uniform(-0.01, 0.01), record=True)
+        # timing jitter: activation times shifted by up to +-10 ms, clipped at t = 0 so no
+        # element starts its twitch before the beat does
+        r = window(m, np.maximum(mesh.activation + rng.uniform(-0.01, 0.01), 0.0), record=True)
         samples += r["samples"]
         log(f"[warm start] training beat {b}: newton {r['newton']} krylov {r['krylov']} {r['wall']:.1f} s")
     t_rec = time.perf_counter() - t0

@@ -23,7 +23,7 @@ EXPORTED = [
     "cmg_pattern", "cmg_set_pressures", "cmg_synchronize", "cmg_stream", "cmg_residual", "cmg_tangent",
     "cmg_element_residuals", "cmg_element_tangents", "cmg_surface_forces", "cmg_cavity_volumes",
     "cmg_mass_matrix", "cmg_residual_device", "cmg_residual_tangent_device", "cmg_element_kernel_device",
-    "cmg_spmv", "cmg_linear_solve", "cmg_set_state", "cmg_get_state", "cmg_set_initial_guess",
+    "cmg_spmv", "cmg_spmv_layout", "cmg_linear_solve", "cmg_set_state", "cmg_get_state", "cmg_set_initial_guess",
     "cmg_static_step", "cmg_newmark_step", "cmg_copy_device",
     "cmg_batch_create", "cmg_batch_destroy", "cmg_batch_configure", "cmg_batch_info_get", "cmg_batch_static_step",
     "cmg_batch_get_nodes", "cmg_batch_set_nodes", "cmg_batch_residual", "cmg_batch_cavity_volumes",
@@ -126,6 +126,7 @@ def load():
         "cmg_residual_tangent_device": (ctypes.c_int, [P, P, dbl, i32, P, P]),
         "cmg_element_kernel_device": (ctypes.c_int, [P, P, dbl, i32]),
         "cmg_spmv": (ctypes.c_int, [P, P, P, P]),
+        "cmg_spmv_layout": (ctypes.c_int, [P, P, P, P, i32, i32]),
         "cmg_linear_solve": (ctypes.c_int, [P, P, P, P, dbl, P]),
         "cmg_set_state": (ctypes.c_int, [P, P, P, P]),
         "cmg_get_state": (ctypes.c_int, [P, P, P, P, P]),

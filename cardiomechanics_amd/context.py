@@ -195,11 +195,15 @@ class MechanicsContext:
         self._check(self.L.cmg_mass_matrix(self.h, _ptr(v)), "cmg_mass_matrix")
         return v
 
-    def spmv(self, values, x):
+    def spmv(self, values, x, lanes=None, chunked=False):
         values = np.ascontiguousarray(values, dtype=np.float64)
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(self.n_dofs)
-        self._check(self.L.cmg_spmv(self.h, _ptr(values), _ptr(x), _ptr(y)), "cmg_spmv")
+        if lanes is None:
+            self._check(self.L.cmg_spmv(self.h, _ptr(values), _ptr(x), _ptr(y)), "cmg_spmv")
+        else:
+            self._check(self.L.cmg_spmv_layout(self.h, _ptr(values), _ptr(x), _ptr(y), lanes, int(chunked)),
+                        "cmg_spmv_layout")
         return y
 
     def linear_solve(self, values, b, x0=None, rtol=1e-10):

@@ -53,6 +53,8 @@ struct cmg_batch {
 
 namespace {
 
+constexpr double kMinDomainLambda = 1.0 / (1 << 20);   // backtracking floor on inverted elements
+
 int upload_flags(cmg_batch* b, const std::vector<int32_t>& f) {
     std::memcpy(b->h_int, f.data(), b->S * sizeof(int32_t));
     CMG_HIP_CHECK(hipMemcpyAsync(b->active.p, b->h_int, b->S * sizeof(int32_t), hipMemcpyHostToDevice, b->c->stream));
@@ -281,7 +283,7 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
     const int S = b->S;
     const int64_t n = b->n;
     const cmg_options& o = c->opt;
-    std::vector<int32_t> run(S, 1), its(S, 0), kry(S, 0), reason(S, 0), corrupt;
+    std::vector<int32_t> run(S, 1), its(S, 0), kry(S, 0), reason(S, 0), zero_tried(S, 0), corrupt;
     std::vector<double> lam(S, 1.0), rprev(S, 0.0), r0(S, 0.0), rn(S, 0.0), sn(S, 0.0), dn(S, 0.0), eta(S, 1e-2);
     const double t0 = now_s();
     double t_tan = 0.0, t_kry = 0.0;
@@ -291,17 +293,23 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
         launch_waxpby(S * n, 1.0, b->nodes.as<double>(), 1.0, b->d.as<double>(), b->xe.as<double>(), s);
         CMG_TRY(upload_flags(b, run));
         CMG_TRY(beval(b, time, true, corrupt));
-        std::vector<double> back(S, 0.0);
+        std::vector<double> back(S, 0.0), keep(S, 1.0);
         std::vector<int32_t> eval(S, 0);
-        bool any_back = false;
+        bool any_back = false, any_zero = false;
         for (int i = 0; i < S; i++) {
             if (!run[i]) continue;
             if (corrupt[i] >= 0) {
                 st[i].corrupt_element = corrupt[i];
-                if (its[i] > 0 && lam[i] > 1.0 / 64) {
+                if (its[i] > 0 && lam[i] > kMinDomainLambda) {
                     lam[i] *= 0.5;
                     back[i] = -lam[i];
                     any_back = true;
+                } else if (its[i] == 0 && !zero_tried[i]) {
+                    // extrapolated initial guess inverts an element: restart from a zero increment
+                    zero_tried[i] = 1;
+                    st[i].guess_source = 4;
+                    keep[i] = 0.0;
+                    any_zero = true;
                 } else {
                     reason[i] = -6;
                     run[i] = 0;
@@ -310,6 +318,7 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
                 eval[i] = 1;
             }
         }
+        if (any_zero) launch_bscale(S, n, coefs(keep), b->d.as<double>(), s);
         if (any(eval)) {
             CMG_TRY(upload_flags(b, eval));
             bgather_residual(b);
@@ -320,7 +329,13 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
         for (int i = 0; i < S; i++) {
             if (!eval[i]) continue;
             rn[i] = std::sqrt(std::max(dots[i], 0.0));
-            if (!std::isfinite(rn[i])) {
+            if (!std::isfinite(dots[i])) {
+                if (its[i] > 0 && lam[i] > kMinDomainLambda) {   // overflow in the laws: a domain error
+                    lam[i] *= 0.5;
+                    back[i] = -lam[i];
+                    any_back = true;
+                    continue;
+                }
                 reason[i] = -4;
                 run[i] = 0;
                 continue;

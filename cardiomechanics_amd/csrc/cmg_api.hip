@@ -303,7 +303,13 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     for (int64_t r = 0; r < N; r++)
         for (int64_t k = c->h_rowptr[r]; k < c->h_rowptr[r + 1]; k++) blkrow[k] = (int32_t)r;
     CMG_TRY(upload(c->rowptr, c->h_rowptr.data(), N + 1, s));
-    CMG_TRY(upload(c->colind, c->h_colind.data(), nnzb, s));
+    {
+        // padded by kSpmvPadBlocks entries for the chunked SpMV (read, never used)
+        std::vector<int32_t> colpad(c->h_colind);
+        colpad.resize(nnzb + kSpmvPadBlocks, 0);
+        CMG_TRY(upload(c->colind, colpad.data(), colpad.size(), s));
+        CMG_HIP_CHECK(hipStreamSynchronize(s));
+    }
     CMG_TRY(upload(c->blkrow, blkrow.data(), nnzb, s));
     // -- assembly contributor lists (deterministic order: solid elements, then surfaces)
     {
@@ -391,7 +397,8 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
         launch_fill(c->n, 0.0, b->as<double>(), s);
     CMG_TRY(alloc(c->fe, (size_t)E * 3 * npe * sizeof(double)));
     CMG_TRY(alloc(c->Ke, (size_t)E * 9 * npe * npe * (c->fp32 ? sizeof(float) : sizeof(double))));
-    CMG_TRY(alloc(c->vals, (size_t)nnzb * 9 * (c->fp32 ? sizeof(float) : sizeof(double))));
+    CMG_TRY(alloc(c->vals, (size_t)(nnzb + kSpmvPadBlocks) * 9 * (c->fp32 ? sizeof(float) : sizeof(double))));
+    launch_fill_int((nnzb + kSpmvPadBlocks) * 9 * (c->fp32 ? 1 : 2), 0, c->vals.as<int>(), s);   // defined padding
     CMG_TRY(alloc(c->dinv, (size_t)N * 9 * sizeof(double)));
     CMG_TRY(alloc(c->corrupt, sizeof(int)));
     c->gm_m = std::max(8, std::min(c->opt.krylov_restart, 1000));
@@ -568,7 +575,8 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
     double bnorm;
     CMG_TRY(norm2(c, b, &bnorm));
     // r = b - A x
-    launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s);
+    launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s, c->spmv_lanes,
+                c->spmv_chunked);
     launch_waxpby(n, 1.0, b, -1.0, w, V, s);
     double beta;
     CMG_TRY(norm2(c, V, &beta));
@@ -593,7 +601,8 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
             double* vj = V + (int64_t)j * n;
             double* vn = V + (int64_t)(j + 1) * n;
             launch_block_jacobi_apply(c->N, c->dinv.as<double>(), vj, z, s);
-            launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, z, vn, s);
+            launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, z, vn, s, c->spmv_lanes,
+                        c->spmv_chunked);
             // pass 1: h_i = v_i . w (i <= j) and ||w||^2 in one multi-dot
             CMG_TRY(multidot(c, j + 2, V, n, vn, h.data()));
             const double wn2 = h[j + 1];
@@ -648,7 +657,8 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
         launch_block_jacobi_apply(c->N, c->dinv.as<double>(), w, z, s);
         launch_axpby(n, 1.0, z, 1.0, x, s);
         // true residual
-        launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s);
+        launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s, c->spmv_lanes,
+                c->spmv_chunked);
         launch_waxpby(n, 1.0, b, -1.0, w, V, s);
         CMG_TRY(norm2(c, V, &beta));
         st->iterations = total;
@@ -688,7 +698,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
     launch_block_jacobi_setup(c->N, rp, ci, vals, c->fp32, c->dinv.as<double>(), s);
     double bnorm;
     CMG_TRY(norm2(c, b, &bnorm));
-    launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s);
+    launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s, c->spmv_lanes, c->spmv_chunked);
     launch_waxpby(n, 1.0, b, -1.0, v, r, s);
     double rn;
     CMG_TRY(norm2(c, r, &rn));
@@ -717,7 +727,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
         }
         rho = rho_new;
         launch_block_jacobi_apply(c->N, c->dinv.as<double>(), p, ph, s);
-        launch_spmv(c->N, rp, ci, vals, c->fp32, ph, v, s);
+        launch_spmv(c->N, rp, ci, vals, c->fp32, ph, v, s, c->spmv_lanes, c->spmv_chunked);
         double d0;
         CMG_TRY(multidot(c, 1, rh, 0, v, &d0));
         bool breakdown = !(std::fabs(d0) > 0.0) || !std::isfinite(d0);
@@ -725,7 +735,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
             alpha = rho / d0;
             launch_waxpby(n, 1.0, r, -alpha, v, sv, s);   // s = r - alpha v
             launch_block_jacobi_apply(c->N, c->dinv.as<double>(), sv, sh, s);
-            launch_spmv(c->N, rp, ci, vals, c->fp32, sh, t, s);
+            launch_spmv(c->N, rp, ci, vals, c->fp32, sh, t, s, c->spmv_lanes, c->spmv_chunked);
             launch_multidot(n, 3, rh, n, t, part, s);      // (rh,t) (s,t) (t,t)
             launch_reduce_partials(nb, 3, part, dh, s);
             launch_multidot(n, 2, rh, n, sv, part, s);     // (rh,s) (s,s)
@@ -745,7 +755,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
         }
         if (rn <= target || breakdown) {
             // the recurrence residual drifts from b - A x: confirm, restart if needed
-            launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s);
+            launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s, c->spmv_lanes, c->spmv_chunked);
             launch_waxpby(n, 1.0, b, -1.0, v, r, s);
             CMG_TRY(norm2(c, r, &rn));
             if (rn <= target || ++restarts > 50 || !std::isfinite(rn)) break;
@@ -759,7 +769,41 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
     return CMG_SUCCESS;
 }
 
+// SpMV layout: the fastest of 4 / 8 / 16 lanes per block row, one block or one
+// 144-byte chunk per step, on this matrix (HIP events, one warm and two timed products
+// each), chosen once per context
+int tune_spmv(cmg_ctx* c, const void* vals, const double* b) {
+    hipStream_t s = c->stream;
+    hipEvent_t e0, e1;
+    CMG_HIP_CHECK(hipEventCreate(&e0));
+    CMG_HIP_CHECK(hipEventCreate(&e1));
+    const int lanes[3] = {4, 8, 16};
+    float best = 1e30f;
+    const int64_t* rp = c->rowptr.as<int64_t>();
+    const int32_t* ci = c->colind.as<int32_t>();
+    for (int chunked = 0; chunked < 2; chunked++)
+        for (int l : lanes) {
+            launch_spmv(c->N, rp, ci, vals, c->fp32, b, c->tmpA.as<double>(), s, l, chunked);
+            CMG_HIP_CHECK(hipEventRecord(e0, s));
+            for (int k = 0; k < 2; k++) launch_spmv(c->N, rp, ci, vals, c->fp32, b, c->tmpA.as<double>(), s, l, chunked);
+            CMG_HIP_CHECK(hipEventRecord(e1, s));
+            CMG_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            c->spmv_ms[chunked * 3 + (l == 4 ? 0 : l == 8 ? 1 : 2)] = 0.5 * ms;
+            if (ms < best) {
+                best = ms;
+                c->spmv_lanes = l;
+                c->spmv_chunked = chunked != 0;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return CMG_SUCCESS;
+}
+
 int krylov(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol, int maxit, cmg_solve_stats* st) {
+    if (c->spmv_lanes == 0) CMG_TRY(tune_spmv(c, vals, b));
     if (c->opt.krylov_method == 1) return bicgstab(c, vals, b, x, rtol, maxit, st);
     return gmres(c, vals, b, x, rtol, maxit, st);
 }
@@ -918,17 +962,25 @@ int cmg_mass_matrix(cmg_ctx* c, double* values) {
     return CMG_SUCCESS;
 }
 
-int cmg_spmv(cmg_ctx* c, const double* values, const double* x, double* y) {
+int cmg_spmv_layout(cmg_ctx* c, const double* values, const double* x, double* y, int32_t lanes, int32_t chunked) {
     if (!c || !values || !x || !y) return fail("cmg_spmv: null argument");
+    if (lanes != 4 && lanes != 8 && lanes != 16) return fail("cmg_spmv_layout: lanes must be 4, 8 or 16");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
     const size_t vb = c->n * sizeof(double);
-    DevBuf v64;
-    CMG_TRY(upload(v64, values, (size_t)c->nnzb * 9, c->stream));
+    DevBuf v64;   // padded for the chunked layout
+    CMG_TRY(alloc(v64, (size_t)(c->nnzb + kSpmvPadBlocks) * 9 * sizeof(double)));
+    launch_fill((c->nnzb + kSpmvPadBlocks) * 9, 0.0, v64.as<double>(), c->stream);
+    CMG_HIP_CHECK(hipMemcpyAsync(v64.p, values, (size_t)c->nnzb * 9 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CMG_HIP_CHECK(hipMemcpyAsync(c->tmpA.p, x, vb, hipMemcpyHostToDevice, c->stream));
     launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), v64.p, false, c->tmpA.as<double>(),
-                c->tmpB.as<double>(), c->stream);
+                c->tmpB.as<double>(), c->stream, lanes, chunked != 0);
     CMG_HIP_CHECK(hipMemcpyAsync(y, c->tmpB.p, vb, hipMemcpyDeviceToHost, c->stream));
     CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
     return CMG_SUCCESS;
+}
+
+int cmg_spmv(cmg_ctx* c, const double* values, const double* x, double* y) {
+    return cmg_spmv_layout(c, values, x, y, 8, 0);
 }
 
 int cmg_linear_solve(cmg_ctx* c, const double* values, const double* b, double* x, double rtol, cmg_solve_stats* st) {
@@ -983,6 +1035,10 @@ int cmg_set_initial_guess(cmg_ctx* c, const double* inc) {
 
 namespace {
 
+// smallest backtracking factor when a Newton step inverts an element (the reference's
+// line search goes down to 1e-12 on a domain error)
+constexpr double kMinDomainLambda = 1.0 / (1 << 20);
+
 // ||r(d)|| of the Newmark residual at the current increment d (infinite when an
 // element inverts)
 int residual_norm_at(cmg_ctx* c, double time, double cm, double* out) {
@@ -1020,6 +1076,7 @@ int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
     st->corrupt_element = -1;
     double eta = 1e-2;
     double lambda = 1.0;
+    bool zero_tried = false;
     int it = 0;
     for (;; it++) {
         const double ta = now_s();
@@ -1028,10 +1085,19 @@ int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
         int rc = eval_elements(c, xe, time, true, true, &corrupt);
         if (rc != CMG_SUCCESS) {
             st->corrupt_element = corrupt;
-            if (it > 0 && lambda > 1.0 / 64) {
+            if (it > 0 && lambda > kMinDomainLambda) {
                 // step into an inverted state: backtrack (SNESSetFunctionDomainError analogue)
                 lambda *= 0.5;
                 launch_axpby(n, -lambda, delta, 1.0, d, s);
+                it--;
+                continue;
+            }
+            if (it == 0 && !zero_tried) {
+                // the extrapolated initial guess inverts an element: start from a zero increment
+                // (the current configuration is valid)
+                zero_tried = true;
+                st->guess_source = 4;
+                launch_fill(n, 0.0, d, s);
                 it--;
                 continue;
             }
@@ -1047,6 +1113,12 @@ int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
         CMG_TRY(norm2(c, r, &rnorm));
         t_tan += now_s() - ta;
         if (!std::isfinite(rnorm)) {
+            if (it > 0 && lambda > kMinDomainLambda) {   // overflow in the laws: a domain error too
+                lambda *= 0.5;
+                launch_axpby(n, -lambda, delta, 1.0, d, s);
+                it--;
+                continue;
+            }
             st->converged_reason = -4;
             break;
         }

@@ -60,6 +60,9 @@ struct cmg_ctx {
     int64_t N = 0, E = 0, S = 0, n = 0;   // nodes, elements, surfaces, dofs
     int n_groups = 0;
     int n_mats = 0;
+    int spmv_lanes = 0;         // lanes per block row of the SpMV, 0 until timed on the first solve
+    bool spmv_chunked = false;  // 144-byte value chunks instead of one block per step
+    double spmv_ms[6] = {0, 0, 0, 0, 0, 0};   // timed SpMV: one block at 4/8/16 lanes, chunks at 4/8/16
     bool fp32 = false;
     // host copies needed for maps / exports
     std::vector<uint8_t> h_bc;

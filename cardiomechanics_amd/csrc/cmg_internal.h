@@ -97,8 +97,11 @@ void launch_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* cont
                           hipStream_t s, const SetStrides& b = SetStrides());   // strides: Ke, Ks, vals
 
 // ---- launchers (linalg_kernels.hip) ----
+// lanes per block row: 4, 8 or 16; chunked: 144-byte value chunks (needs 16-byte
+// aligned values and columns padded by kSpmvPadBlocks blocks)
+constexpr int kSpmvPadBlocks = 4;
 void launch_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals, bool fp32,
-                 const double* x, double* y, hipStream_t s);
+                 const double* x, double* y, hipStream_t s, int lanes = 8, bool chunked = false);
 void launch_spmv_scalar_mass(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval,
                              const uint8_t* bcmask, const double* x, double* y, hipStream_t s);
 void launch_block_jacobi_setup(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals,

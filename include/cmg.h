@@ -139,7 +139,9 @@ typedef struct cmg_step_stats {
     double krylov_seconds;
     int32_t guess_source;        /* Newton initial guess: 0 predictor / rescaled previous increment,
                                     1 cmg_set_initial_guess, 2 proposed guess kept (its residual was
-                                    below the predictor's), 3 proposed guess rejected */
+                                    below the predictor's), 3 proposed guess rejected, 4 the
+                                    initial guess inverted an element and Newton restarted from a
+                                    zero increment */
     int32_t reserved_s;
     double guess_residual_ratio; /* ||r(proposed)|| / ||r(predictor)|| when a guess was proposed */
 } cmg_step_stats;
@@ -203,6 +205,11 @@ int cmg_copy_device(cmg_ctx* ctx, const double* src_dev, double* dst_dev, int64_
 
 /* ---- linear algebra on the context's pattern ---------------------------- */
 int cmg_spmv(cmg_ctx* ctx, const double* values, const double* x, double* y);
+/* the same product with an explicit kernel layout: lanes per block row 4, 8 or 16;
+ * chunked 1 reads the values in 144-byte chunks (the Krylov solvers pick the fastest
+ * layout on their first solve) */
+int cmg_spmv_layout(cmg_ctx* ctx, const double* values, const double* x, double* y, int32_t lanes,
+                    int32_t chunked);
 /* BiCGStab or GMRES(m) (cmg_options.krylov_method), right preconditioned with 3x3
  * nodal block Jacobi; replaces the KSP/PC of CBSolverNewmarkBeta.cpp:166-196 (a
  * Krylov solve instead of LU). */

@@ -17,7 +17,7 @@ from cardiomechanics_amd.context import MechanicsContext, ParameterBatch
 
 pytestmark = pytest.mark.gpu
 
-TIGHT = dict(newton_atol=1e-11, newton_rtol=1e-12, newton_stol=1e-15, krylov_maxit=20000)
+TIGHT = dict(newton_atol=1e-11, newton_rtol=1e-12, newton_stol=1e-15, krylov_maxit=20000, newton_maxit=200)
 NO_PTAN = dict(TIGHT, pressure_tangent=0)   # open pressure face (cube): keep the batched matrices symmetric
 
 

@@ -116,7 +116,7 @@ def test_warm_start_keeps_the_converged_solution_and_reports_its_gain():
     for b in range(3):
         m = MT.lv_holzapfel(tension_max=60e3 * rng.uniform(0.85, 1.15))
         mb = M.Mesh(coords=mesh.coords, elements=mesh.elements, bases=mesh.bases, bc=mesh.bc,
-                    material=mesh.material, activation=mesh.activation + rng.uniform(-0.01, 0.01),
+                    material=mesh.material, activation=np.maximum(mesh.activation + rng.uniform(-0.01, 0.01), 0.0),
                     surfaces=mesh.surfaces, surface_id=mesh.surface_id)
         samples += run_window(mb, m, steps, dt, record=True)[1]
     with MechanicsContext(mesh, [base]) as tctx:

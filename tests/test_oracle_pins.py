@@ -3,6 +3,7 @@ holds golden numbers for this path).  Each check is chosen so that a plausible
 restatement mistake -- a dropped term, a wrong sign, a transposed operand, a wrong
 midside-node order -- fails it."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -369,3 +370,31 @@ def test_mlp_oracle_against_torch_autograd_and_finite_differences():
         e[k] = h
         fd = (mlp.loss(w + e, X, T, H) - mlp.loss(w - e, X, T, H)) / (2 * h)
         assert abs(fd - go[k]) <= 1e-7 * max(1.0, abs(go[k]))
+
+
+def test_lv_generator_orients_the_endocardium_like_the_reference_mesh():
+    """Endocardial normals n = (b-a) x (c-a) point into the cavity (towards the long
+    axis) in the reference's ellipsoid.sur and in the generator, so a negative
+    pressure amplitude inflates both (the oracle: the cavity volume grows)."""
+    from cardiomechanics_amd import materials as MT
+    from cardiomechanics_amd import mesh as M
+    from oracle.drivers import CpuMechanics
+
+    def inward_fraction(X, S, axis_xy):
+        a, b, c = X[S[:, 0]], X[S[:, 1]], X[S[:, 2]]
+        n = np.cross(b - a, c - a)
+        rad = (a + b + c) / 3 - np.array([axis_xy[0], axis_xy[1], 0.0])
+        rad[:, 2] = 0.0
+        return np.mean(np.einsum("ij,ij->i", n, rad) < 0)
+
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    z = np.load(os.path.join(golden, "ellipsoid_subset.npz"))
+    assert inward_fraction(z["coords"], z["surfaces"], (0.0, 0.0)) == 1.0
+    mesh = M.lv_mesh(dims=(2, 4, 8))
+    assert inward_fraction(mesh.coords, mesh.surfaces, (0.0, 0.0)) == 1.0
+    cpu = CpuMechanics(mesh, [MT.cube_guccione()])
+    v0 = cpu.cavity_volumes()[0]
+    cpu.pressures[:] = -500.0
+    cpu.static_step(0.0, 1.0)
+    cpu.static_step(1.0, 1.0)
+    assert v0 < 0 and cpu.cavity_volumes()[0] < v0      # |V| grows
