@@ -43,8 +43,8 @@ def main():
     ap.add_argument("--elements", type=int, default=200_000)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--beat-steps", type=int, default=2, help="Newmark steps of the beat to time (0: skip)")
-    ap.add_argument("--krylov", default="gmres,bicgstab",
-                    help="Krylov methods to time the beat steps with (comma list of gmres, bicgstab)")
+    ap.add_argument("--krylov", default="bicgstab,bicgstab_graph",
+                    help="Krylov methods to time the beat steps with (comma list of bicgstab, bicgstab_graph, gmres)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="CPU baseline sample: repeat 20k-element chunks of the mesh for about this long")
     ap.add_argument("--no-cpu", action="store_true")
@@ -125,7 +125,7 @@ def main():
 
     # kernel variants of the fused residual + tangent element kernel; the fastest is the headline
     variants = {}
-    for var in (2, 3, 1):
+    for var in (2, 4, 3, 1):
         ctx = MechanicsContext(mesh, mats, precision=prec, element_variant=var)
         if xd is None:
             xd = DeviceArray(x.ravel())
@@ -167,7 +167,7 @@ def main():
     if args.beat_steps > 0:
         beat = {}
         for name in args.krylov.split(","):
-            method = {"gmres": 0, "bicgstab": 1}[name.strip()]
+            method = {"gmres": 0, "bicgstab_graph": 1, "bicgstab": 2}[name.strip()]
             bctx = MechanicsContext(mesh, mats, precision=prec, element_variant=best, krylov_method=method)
             plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
             dt = 1e-3
@@ -293,7 +293,7 @@ def warm_start(args, prec):
 
     mesh = M.lv_mesh(args.ws_elements)
     dt, steps = 1e-3, args.ws_steps
-    tol = dict(precision=prec, newton_atol=1e-8, newton_rtol=1e-8, newton_stol=1e-8)
+    tol = dict(precision=prec, newton_atol=1e-8, newton_rtol=1e-8, newton_stol=1e-8, newton_maxit=100)
     log(f"[warm start] {mesh.n_elements} T10 elements, {steps} steps of {dt * 1e3:.0f} ms, "
         f"{args.ws_train} training beats")
 
@@ -301,9 +301,21 @@ def warm_start(args, prec):
         msh = M.Mesh(coords=mesh.coords, elements=mesh.elements, bases=mesh.bases, bc=mesh.bc,
                      material=mesh.material, activation=act, surfaces=mesh.surfaces, surface_id=mesh.surface_id)
         plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
-        newton = krylov = kept = 0
+        newton = krylov = kept = halved = 0
         wall = 0.0
         nodes = []
+
+        def advance(ctx, t_end, h, depth):
+            # a failed step halves dt (CBTiming, CBTiming.cpp:90-100): two half steps
+            plug.apply(t_end, h)
+            ctx.set_pressures(plug.values(ctx.n_groups))
+            rc, st = ctx.newmark_step(t_end, h)
+            if rc == 2:
+                return [st]
+            if depth >= 5:
+                raise RuntimeError(f"Newmark step at t={t_end} failed at dt={h}: {st.as_dict()}")
+            return advance(ctx, t_end - 0.5 * h, 0.5 * h, depth + 1) + advance(ctx, t_end, 0.5 * h, depth + 1)
+
         with MechanicsContext(msh, [m], **tol) as ctx:
             net = WarmStartNet(ctx, seed=5)
             if weights is not None:
@@ -321,19 +333,21 @@ def warm_start(args, prec):
                 t0 = time.perf_counter()
                 if weights is not None and k > 2:
                     net.propose(t, rate, bf16=args.ws_bf16)
-                rc, st = ctx.newmark_step(t, dt)
+                sts = advance(ctx, t, dt, 0)
                 wall += time.perf_counter() - t0
-                if rc != 2:
-                    raise RuntimeError(f"Newmark step {k} failed: {st.as_dict()}")
                 if record:
-                    net.record_after_step()
-                newton += st.newton_iterations
-                krylov += st.krylov_iterations
-                kept += st.guess_source == 2
+                    if len(sts) == 1:
+                        net.record_after_step()
+                    else:   # the increment history now holds a partial step: not a training pair
+                        net._pending = None
+                halved += len(sts) > 1
+                newton += sum(st.newton_iterations for st in sts)
+                krylov += sum(st.krylov_iterations for st in sts)
+                kept += sts[0].guess_source == 2
                 nodes.append(ctx.get_state()["nodes"])
             samples = net.samples
             net.close()
-        return dict(newton=newton, krylov=krylov, wall=wall, kept=kept, nodes=nodes, samples=samples)
+        return dict(newton=newton, krylov=krylov, wall=wall, kept=kept, halved=halved, nodes=nodes, samples=samples)
 
     rng = np.random.default_rng(0)
     samples = []
@@ -344,7 +358,8 @@ def warm_start(args, prec):
         # element starts its twitch before the beat does
         r = window(m, np.maximum(mesh.activation + rng.uniform(-0.01, 0.01), 0.0), record=True)
         samples += r["samples"]
-        log(f"[warm start] training beat {b}: newton {r['newton']} krylov {r['krylov']} {r['wall']:.1f} s")
+        log(f"[warm start] training beat {b}: newton {r['newton']} krylov {r['krylov']} halved {r['halved']} "
+            f"{r['wall']:.1f} s")
     t_rec = time.perf_counter() - t0
     with MechanicsContext(mesh, [MT.lv_holzapfel()], **tol) as tctx:
         net = WarmStartNet(tctx, seed=5)
@@ -365,8 +380,8 @@ def warm_start(args, prec):
            "n_gpus": 1, "higher_is_better": False, "dtype": "f64 (network f32" + ("/bf16" if args.ws_bf16 else "") + ")",
            "config": {"workload": "LV Newmark beat window, Holzapfel-Ogden + active tension", "elements": mesh.n_elements,
                       "steps": steps, "dt": dt, "training_beats": args.ws_train, "epochs": args.ws_epochs},
-           "plain": {k: plain[k] for k in ("newton", "krylov", "wall")},
-           "warm": {k: warm[k] for k in ("newton", "krylov", "wall", "kept")},
+           "plain": {k: plain[k] for k in ("newton", "krylov", "wall", "halved")},
+           "warm": {k: warm[k] for k in ("newton", "krylov", "wall", "kept", "halved")},
            "speedup": plain["wall"] / warm["wall"], "max_rel_difference": max(errs),
            "training": {"record_seconds": t_rec, "train_seconds": t_train, "loss_first": hist[0],
                         "loss_last": hist[-1], "samples": len(samples)}}

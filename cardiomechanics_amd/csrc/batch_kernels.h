@@ -24,6 +24,31 @@ struct BatchCG {
     int64_t part_ld = 0;
 };
 
+// Device state of S right block-Jacobi preconditioned BiCGStab solves (van der
+// Vorst 1992), scalars on the device: one iteration is eight launches and no host
+// synchronisation.  done: 0 running, 1 converged by the recurrence residual,
+// 2 breakdown (the host confirms with the true residual and restarts when needed).
+struct BatchBiCG {
+    int S = 0;
+    double *x = nullptr, *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr;   // [S][n]
+    double *s = nullptr, *t = nullptr, *ph = nullptr, *sh = nullptr;              // [S][n]
+    double *rho = nullptr, *alpha = nullptr, *omega = nullptr, *beta = nullptr;    // [S]
+    double *rr = nullptr, *tol2 = nullptr;                                        // [S]
+    int32_t *done = nullptr, *it = nullptr, *fin = nullptr;                       // [S]
+    double* part = nullptr;                                                       // [S][part_ld]
+    int64_t part_ld = 0;
+};
+int64_t bicg_part_ld(int64_t N);   // partial-sum slots per set the kernels need
+
+// r must hold b - A x: r^ = r, p = r, rho = r.r, tol2 = (rtol ||b||)^2 for the active sets
+void launch_bicg_init(const BatchBiCG& k, int64_t N, const double* b, const double* rtol, const int32_t* active,
+                      hipStream_t s);
+// one BiCGStab iteration of every running set; chunked: 144-byte value chunks (values
+// 16-byte aligned per set and padded, as for launch_spmv)
+void launch_bicg_iteration(const BatchBiCG& k, int lanes, bool chunked, int64_t N, const int64_t* rowptr,
+                           const int32_t* colind, const void* vals, bool fp32, int64_t set_vals, const double* dinv,
+                           hipStream_t s);
+
 int batch_dot_blocks(int64_t n);
 int batch_spmv_blocks(int64_t N, int lanes);
 int batch_node_blocks(int64_t N);

@@ -252,7 +252,319 @@ __global__ __launch_bounds__(kThreads) void k_bscale(int64_t n, SetCoefs a, doub
     y[set * n + i] *= c;
 }
 
+
+// ---------------------------------------------------------------------------
+// BiCGStab with device scalars (BatchBiCG)
+// ---------------------------------------------------------------------------
+constexpr int kBicgDots = 5;
+
+// y = A u for the rows of this block (LANES lanes per block row); DOTS = 1: partial
+// rh.y; DOTS = 5: partials y.w, y.y, rh.y, rh.w, w.w (y = t, w = s)
+// this lane's share of row r of A u: one block per step, or 144-byte chunks of
+// 2 (FP64) / 4 (FP32) blocks read with 16-byte loads (values and columns padded)
+template <typename VT, int LANES, bool CHUNKED>
+__device__ __forceinline__ void row_part(int64_t r, int sub, const int64_t* rowptr, const int32_t* colind,
+                                         const VT* vals, const double* u, double& s0, double& s1, double& s2) {
+    const int64_t k0 = rowptr[r], k1 = rowptr[r + 1];
+    if constexpr (!CHUNKED) {
+        for (int64_t k = k0 + sub; k < k1; k += LANES) {
+            const int64_t c = colind[k];
+            const VT* a = vals + 9 * k;
+            const double x0 = u[3 * c], x1 = u[3 * c + 1], x2 = u[3 * c + 2];
+            s0 += (double)a[0] * x0 + (double)a[1] * x1 + (double)a[2] * x2;
+            s1 += (double)a[3] * x0 + (double)a[4] * x1 + (double)a[5] * x2;
+            s2 += (double)a[6] * x0 + (double)a[7] * x1 + (double)a[8] * x2;
+        }
+    } else {
+        constexpr int BPC = sizeof(VT) == 8 ? 2 : 4;
+        const uint4* V4 = reinterpret_cast<const uint4*>(vals);
+        for (int64_t ch = k0 / BPC + sub; ch * BPC < k1; ch += LANES) {
+            uint4 raw[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) raw[q] = V4[9 * ch + q];
+            VT v[9 * BPC];
+            __builtin_memcpy(v, raw, sizeof(v));
+            int32_t cols[BPC];
+            if constexpr (BPC == 2) {
+                const int2 c2 = reinterpret_cast<const int2*>(colind)[ch];
+                cols[0] = c2.x;
+                cols[1] = c2.y;
+            } else {
+                const int4 c4 = reinterpret_cast<const int4*>(colind)[ch];
+                cols[0] = c4.x;
+                cols[1] = c4.y;
+                cols[2] = c4.z;
+                cols[3] = c4.w;
+            }
+#pragma unroll
+            for (int b = 0; b < BPC; b++) {
+                const int64_t k = ch * BPC + b;
+                if (k < k0 || k >= k1) continue;
+                const int64_t c = cols[b];
+                const VT* a = v + 9 * b;
+                const double x0 = u[3 * c], x1 = u[3 * c + 1], x2 = u[3 * c + 2];
+                s0 += (double)a[0] * x0 + (double)a[1] * x1 + (double)a[2] * x2;
+                s1 += (double)a[3] * x0 + (double)a[4] * x1 + (double)a[5] * x2;
+                s2 += (double)a[6] * x0 + (double)a[7] * x1 + (double)a[8] * x2;
+            }
+        }
+    }
+}
+
+// y = A u for the rows of this block (LANES lanes per block row); DOTS = 1: partial
+// rh.y; DOTS = 5: partials y.w, y.y, rh.y, rh.w, w.w (y = t, w = s)
+template <typename VT, int LANES, int DOTS, bool CHUNKED>
+__global__ __launch_bounds__(kThreads) void k_bicg_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind,
+                                                        const VT* vals, int64_t set_vals, const double* u, double* y,
+                                                        const double* rh, const double* w, const int32_t* done,
+                                                        double* part, int64_t part_ld) {
+    __shared__ double sh[4];
+    const int64_t set = blockIdx.y;
+    if (done[set]) return;
+    const int64_t n = 3 * N;
+    vals += set * set_vals;
+    u += set * n;
+    y += set * n;
+    rh += set * n;
+    if (w) w += set * n;
+    const int64_t gt = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t r = gt / LANES;
+    const int sub = (int)(gt % LANES);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (r < N) row_part<VT, LANES, CHUNKED>(r, sub, rowptr, colind, vals, u, s0, s1, s2);
+#pragma unroll
+    for (int o = LANES / 2; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, LANES);
+        s1 += __shfl_xor(s1, o, LANES);
+        s2 += __shfl_xor(s2, o, LANES);
+    }
+    double d[DOTS];
+#pragma unroll
+    for (int j = 0; j < DOTS; j++) d[j] = 0.0;
+    if (r < N && sub == 0) {
+        y[3 * r] = s0;
+        y[3 * r + 1] = s1;
+        y[3 * r + 2] = s2;
+        const double yv[3] = {s0, s1, s2};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double hr = rh[3 * r + c];
+            if (DOTS == 1) {
+                d[0] += hr * yv[c];
+            } else {
+                const double wc = w[3 * r + c];
+                d[0] += yv[c] * wc;
+                d[DOTS > 1 ? 1 : 0] += yv[c] * yv[c];
+                d[DOTS > 2 ? 2 : 0] += hr * yv[c];
+                d[DOTS > 3 ? 3 : 0] += hr * wc;
+                d[DOTS > 4 ? 4 : 0] += wc * wc;
+            }
+        }
+    }
+    // one wave-level reduction per value, then a single exchange through LDS
+    __shared__ double shm[DOTS][kThreads / 64];
+#pragma unroll
+    for (int j = 0; j < DOTS; j++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) d[j] += __shfl_xor(d[j], o, 64);
+    if (threadIdx.x % 64 == 0)
+#pragma unroll
+        for (int j = 0; j < DOTS; j++) shm[j][threadIdx.x / 64] = d[j];
+    __syncthreads();
+    if (threadIdx.x < DOTS) {
+        double* out = part + set * part_ld + (int64_t)DOTS * blockIdx.x;
+        out[threadIdx.x] = shm[threadIdx.x][0] + shm[threadIdx.x][1] + shm[threadIdx.x][2] + shm[threadIdx.x][3];
+    }
+    (void)sh;
+}
+
+// z = D^-1 u (node per thread)
+__device__ __forceinline__ void bj3(const double* a, const double* u, double* z) {
+    z[0] = a[0] * u[0] + a[1] * u[1] + a[2] * u[2];
+    z[1] = a[3] * u[0] + a[4] * u[1] + a[5] * u[2];
+    z[2] = a[6] * u[0] + a[7] * u[1] + a[8] * u[2];
+}
+
+__global__ __launch_bounds__(kThreads) void k_bicg_precond_p(BatchBiCG k, int64_t N, const double* dinv) {
+    const int64_t set = blockIdx.y;
+    if (k.done[set]) return;
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const int64_t o = set * 3 * N + 3 * i;
+    bj3(dinv + set * 9 * N + 9 * i, k.p + o, k.ph + o);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bicg_alpha(BatchBiCG k, int nb) {
+    __shared__ double sh[4];
+    const int set = blockIdx.x;
+    if (k.done[set]) return;
+    const double rv = reduce_partials(k.part + (int64_t)set * k.part_ld, nb, 1, 0, sh);
+    if (threadIdx.x == 0) {
+        if (rv != 0.0 && isfinite(rv)) {
+            k.alpha[set] = k.rho[set] / rv;
+        } else {
+            k.done[set] = 2;
+        }
+    }
+}
+
+// s = r - alpha v, sh = D^-1 s
+__global__ __launch_bounds__(kThreads) void k_bicg_s(BatchBiCG k, int64_t N, const double* dinv) {
+    const int64_t set = blockIdx.y;
+    if (k.done[set]) return;
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const double al = k.alpha[set];
+    const int64_t o = set * 3 * N + 3 * i;
+    double sv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        sv[c] = k.r[o + c] - al * k.v[o + c];
+        k.s[o + c] = sv[c];
+    }
+    bj3(dinv + set * 9 * N + 9 * i, sv, k.sh + o);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bicg_omega(BatchBiCG k, int nb) {
+    __shared__ double sh[4];
+    const int set = blockIdx.x;
+    if (k.done[set]) return;
+    const double* part = k.part + (int64_t)set * k.part_ld;
+    double d[kBicgDots];
+    for (int j = 0; j < kBicgDots; j++) d[j] = reduce_partials(part, nb, kBicgDots, j, sh);
+    if (threadIdx.x == 0) {
+        const double ts = d[0], tt = d[1], rht = d[2], rhs = d[3], ss = d[4];
+        const double om = (tt > 0.0) ? ts / tt : 0.0;
+        const double rho_new = rhs - om * rht;
+        const double rr = ss - 2.0 * om * ts + om * om * tt;
+        k.omega[set] = om;
+        k.rr[set] = rr;
+        k.it[set] += 1;
+        int fin = 0;
+        if (rr <= k.tol2[set]) fin = 1;
+        else if (om == 0.0 || rho_new == 0.0 || !isfinite(rho_new) || !isfinite(rr)) fin = 2;
+        if (fin == 0) k.beta[set] = (rho_new / k.rho[set]) * (k.alpha[set] / om);
+        k.rho[set] = rho_new;
+        k.fin[set] = fin;
+    }
+}
+
+// x += alpha ph + omega sh; r = s - omega t; p = r + beta (p - omega v) unless finishing
+__global__ __launch_bounds__(kThreads) void k_bicg_update(BatchBiCG k, int64_t n) {
+    const int64_t set = blockIdx.y;
+    if (k.done[set]) return;
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = set * n + i;
+    const double al = k.alpha[set], om = k.omega[set];
+    k.x[o] += al * k.ph[o] + om * k.sh[o];
+    const double rn = k.s[o] - om * k.t[o];
+    k.r[o] = rn;
+    if (k.fin[set] == 0) k.p[o] = rn + k.beta[set] * (k.p[o] - om * k.v[o]);
+}
+
+__global__ void k_bicg_commit(BatchBiCG k) {
+    const int set = blockIdx.x * blockDim.x + threadIdx.x;
+    if (set >= k.S || k.done[set]) return;
+    if (k.fin[set]) k.done[set] = k.fin[set];
+}
+
+// after r = b - A x: r^ = r, p = r; partials r.r and b.b
+__global__ __launch_bounds__(kThreads) void k_bicg_init(BatchBiCG k, int64_t n, const double* b,
+                                                        const int32_t* active, int nb) {
+    __shared__ double sh[4];
+    const int64_t set = blockIdx.y;
+    if (!active[set]) return;
+    double rr = 0.0, bb = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)nb * kThreads) {
+        const int64_t o = set * n + i;
+        const double rv = k.r[o], bv = b[o];
+        k.rh[o] = rv;
+        k.p[o] = rv;
+        rr += rv * rv;
+        bb += bv * bv;
+    }
+    const double trr = block_sum256(rr, sh);
+    const double tbb = block_sum256(bb, sh);
+    if (threadIdx.x == 0) {
+        double* part = k.part + set * k.part_ld + 2 * blockIdx.x;
+        part[0] = trr;
+        part[1] = tbb;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_bicg_init_reduce(BatchBiCG k, int nb, const double* rtol,
+                                                               const int32_t* active) {
+    __shared__ double sh[4];
+    const int set = blockIdx.x;
+    if (!active[set]) {
+        if (threadIdx.x == 0) {
+            k.done[set] = 1;
+            k.fin[set] = 0;
+        }
+        return;
+    }
+    const double* part = k.part + (int64_t)set * k.part_ld;
+    const double rr = reduce_partials(part, nb, 2, 0, sh);
+    const double bb = reduce_partials(part, nb, 2, 1, sh);
+    if (threadIdx.x == 0) {
+        const double bt = (bb > 0.0) ? bb : 1.0;
+        k.rho[set] = rr;
+        k.rr[set] = rr;
+        k.tol2[set] = rtol[set] * rtol[set] * bt;
+        k.fin[set] = 0;
+        k.done[set] = (rr <= k.tol2[set]) ? 1 : 0;
+    }
+}
+
 }  // namespace
+
+int64_t bicg_part_ld(int64_t N) {
+    return std::max<int64_t>((int64_t)kBicgDots * batch_spmv_blocks(N, 16), 2 * (int64_t)batch_dot_blocks(3 * N)) + 1;
+}
+
+void launch_bicg_init(const BatchBiCG& k, int64_t N, const double* b, const double* rtol, const int32_t* active,
+                      hipStream_t s) {
+    const int nb = batch_dot_blocks(3 * N);
+    hipLaunchKernelGGL(k_bicg_init, dim3(nb, k.S), dim3(kThreads), 0, s, k, 3 * N, b, active, nb);
+    hipLaunchKernelGGL(k_bicg_init_reduce, dim3(k.S), dim3(kThreads), 0, s, k, nb, rtol, active);
+}
+
+void launch_bicg_iteration(const BatchBiCG& k, int lanes, bool chunked, int64_t N, const int64_t* rowptr,
+                           const int32_t* colind, const void* vals, bool fp32, int64_t set_vals, const double* dinv,
+                           hipStream_t s) {
+    const int nbn = batch_node_blocks(N);
+    const int nbs = batch_spmv_blocks(N, lanes);
+    const int64_t n = 3 * N;
+    hipLaunchKernelGGL(k_bicg_precond_p, dim3(nbn, k.S), dim3(kThreads), 0, s, k, N, dinv);
+#define CMG_BSPMV(VT, L, D, U, Y, W)                                                                              \
+    if (chunked)                                                                                                  \
+        hipLaunchKernelGGL((k_bicg_spmv<VT, L, D, true>), dim3(nbs, k.S), dim3(kThreads), 0, s, N, rowptr, colind, \
+                           (const VT*)vals, set_vals, U, Y, k.rh, W, k.done, k.part, k.part_ld);                  \
+    else                                                                                                          \
+        hipLaunchKernelGGL((k_bicg_spmv<VT, L, D, false>), dim3(nbs, k.S), dim3(kThreads), 0, s, N, rowptr,       \
+                           colind, (const VT*)vals, set_vals, U, Y, k.rh, W, k.done, k.part, k.part_ld)
+#define CMG_BSPMV_L(D, U, Y, W)                                       \
+    if (fp32) {                                                       \
+        if (lanes == 4) CMG_BSPMV(float, 4, D, U, Y, W);              \
+        else if (lanes == 16) CMG_BSPMV(float, 16, D, U, Y, W);       \
+        else CMG_BSPMV(float, 8, D, U, Y, W);                         \
+    } else {                                                          \
+        if (lanes == 4) CMG_BSPMV(double, 4, D, U, Y, W);             \
+        else if (lanes == 16) CMG_BSPMV(double, 16, D, U, Y, W);      \
+        else CMG_BSPMV(double, 8, D, U, Y, W);                        \
+    }
+    CMG_BSPMV_L(1, k.ph, k.v, (const double*)nullptr)
+    hipLaunchKernelGGL(k_bicg_alpha, dim3(k.S), dim3(kThreads), 0, s, k, nbs);
+    hipLaunchKernelGGL(k_bicg_s, dim3(nbn, k.S), dim3(kThreads), 0, s, k, N, dinv);
+    CMG_BSPMV_L(kBicgDots, k.sh, k.t, k.s)
+#undef CMG_BSPMV_L
+#undef CMG_BSPMV
+    hipLaunchKernelGGL(k_bicg_omega, dim3(k.S), dim3(kThreads), 0, s, k, nbs);
+    hipLaunchKernelGGL(k_bicg_update, dim3((unsigned)((n + kThreads - 1) / kThreads), k.S), dim3(kThreads), 0, s, k,
+                       n);
+    hipLaunchKernelGGL(k_bicg_commit, dim3((k.S + 63) / 64), dim3(64), 0, s, k);
+}
 
 int batch_dot_blocks(int64_t n) {
     const int64_t b = (n + 8 * kThreads - 1) / (8 * kThreads);

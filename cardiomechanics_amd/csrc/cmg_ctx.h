@@ -62,6 +62,13 @@ struct cmg_ctx {
     int n_mats = 0;
     int spmv_lanes = 0;         // lanes per block row of the SpMV, 0 until timed on the first solve
     bool spmv_chunked = false;  // 144-byte value chunks instead of one block per step
+    // device-scalar BiCGStab: workspace and the graph of `bicg_chunk` iterations
+    cmg::DevBuf bicg_part, bicg_sc, bicg_flags;
+    hipGraphExec_t bicg_graph = nullptr;
+    const void* bicg_graph_x = nullptr;
+    const void* bicg_graph_vals = nullptr;
+    int bicg_chunk = 16;
+    bool bicg_use_graph = true;
     double spmv_ms[6] = {0, 0, 0, 0, 0, 0};   // timed SpMV: one block at 4/8/16 lanes, chunks at 4/8/16
     bool fp32 = false;
     // host copies needed for maps / exports

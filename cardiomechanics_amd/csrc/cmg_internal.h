@@ -71,7 +71,8 @@ struct SurfArgs {
 
 // ---- launchers (element_kernels.hip) ----
 void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s);
-// variant: 1 = per-lane quadrature points, 2 = shared stage (LDS), 3 = shared stage, two node passes
+// variant: 1 = per-lane quadrature points, 2 = shared stage (LDS), 3 = shared stage, two node passes,
+// 4 = shared stage with two lanes (two waves) per element and spatial pair
 void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s, int variant);
 void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s);
 void launch_t10_shape(int64_t E, const int32_t* conn_aos, const double* X, double* dNdX_soa, double* w, hipStream_t s);

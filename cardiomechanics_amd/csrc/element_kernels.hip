@@ -17,6 +17,8 @@
 // the 9 entries M^{ij} it needs, so no LDS and no barriers are used.
 #include <climits>
 
+#include <type_traits>
+
 #include "cmg_internal.h"
 
 namespace cmg {
@@ -468,7 +470,13 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
             const TT mg1 = M[3] * G[b][0] + M[4] * G[b][1] + M[5] * G[b][2];
             const TT mg2 = M[6] * G[b][0] + M[7] * G[b][1] + M[8] * G[b][2];
 #pragma unroll
-            for (int aa = 0; aa < NPE; aa++) acc[aa * NPE + b] += G[aa][0] * mg0 + G[aa][1] * mg1 + G[aa][2] * mg2;
+            for (int aa = 0; aa < NPE; aa++) {
+                // three chained fused multiply-adds into the accumulator
+                TT t = acc[aa * NPE + b];
+                t = fma(G[aa][0], mg0, t);
+                t = fma(G[aa][1], mg1, t);
+                acc[aa * NPE + b] = fma(G[aa][2], mg2, t);
+            }
         }
     }
     const double V = a.w[e];
@@ -491,12 +499,14 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
 // data the tangent needs -- and park it in LDS.  Stage B: 63 lanes (7 elements x
 // 9 spatial pairs (i, j)) build M^{ij} from LDS and contract it with G into
 // K(a,b)_ij.  NSPLIT > 1 splits the node range a into NSPLIT passes so fewer
-// accumulators stay live (more passes, less register pressure).
+// accumulators stay live (more passes, less register pressure).  BSPLIT > 1
+// gives each (element, i, j) BSPLIT lanes, one per part of the node range b, in a
+// workgroup of BSPLIT waves: fewer accumulators per lane, more waves per SIMD.
 // ---------------------------------------------------------------------------
 constexpr int kQRec = 51;   // Ft(9) BBt(9) S(9) Z(9) Y|X(9) czz codot ceq d00 d11 mfs
 
-template <int NPE, int NQP, typename TT, int NSPLIT>
-__global__ __launch_bounds__(64) void k_residual_tangent_v2(const ElemArgs a0) {
+template <int NPE, int NQP, typename TT, int NSPLIT, int BSPLIT = 1>
+__global__ __launch_bounds__(64 * BSPLIT) void k_residual_tangent_v2(const ElemArgs a0) {
     constexpr int EPW = kTanElemsPerWave;
     ElemArgs a;
     if (!batch_select(a0, a)) return;   // uniform per workgroup: no barrier below is skipped by part of it
@@ -573,8 +583,9 @@ __global__ __launch_bounds__(64) void k_residual_tangent_v2(const ElemArgs a0) {
     }
     __syncthreads();
     // ---- stage B ----
-    const int slot = lane / 9;
-    const int ij = lane - 9 * slot;
+    const int bl = lane % 64, bh = lane / 64;   // lane in its wave; part of the b range
+    const int slot = bl / 9;
+    const int ij = bl - 9 * slot;
     const int ii = ij / 3, jj = ij - 3 * (ij / 3);
     const int64_t e = e0 + slot;
     if (slot >= EPW || e >= E) return;
@@ -590,11 +601,13 @@ __global__ __launch_bounds__(64) void k_residual_tangent_v2(const ElemArgs a0) {
     const double V = a.w[e];
     TT* Ke = static_cast<TT*>(a.Ke) + e * (9 * NPE * NPE);
     constexpr int AH = NPE / NSPLIT;
+    constexpr int BH = NPE / BSPLIT;
+    const int b0 = bh * BH;
 #pragma unroll 1
     for (int half = 0; half < NSPLIT; half++) {
-        TT acc[AH * NPE];
+        TT acc[AH * BH];
 #pragma unroll
-        for (int r = 0; r < AH * NPE; r++) acc[r] = (TT)0;
+        for (int r = 0; r < AH * BH; r++) acc[r] = (TT)0;
 #pragma unroll 1
         for (int q = 0; q < NQP; q++) {
             const TT* rec = sQ[slot][q];
@@ -654,23 +667,39 @@ __global__ __launch_bounds__(64) void k_residual_tangent_v2(const ElemArgs a0) {
             for (int k = 0; k < NPE; k++)
 #pragma unroll
                 for (int r = 0; r < 3; r++) G[k][r] = sG[slot][q][3 * k + r];
+            // b range of this lane with a compile-time offset (no dynamic register indexing)
+            auto contract = [&](auto B0) {
+                constexpr int kB0 = decltype(B0)::value;
 #pragma unroll
-            for (int b = 0; b < NPE; b++) {
-                const TT mg0 = M[0] * G[b][0] + M[1] * G[b][1] + M[2] * G[b][2];
-                const TT mg1 = M[3] * G[b][0] + M[4] * G[b][1] + M[5] * G[b][2];
-                const TT mg2 = M[6] * G[b][0] + M[7] * G[b][1] + M[8] * G[b][2];
+                for (int bb = 0; bb < BH; bb++) {
+                    const int b = kB0 + bb;
+                    const TT mg0 = M[0] * G[b][0] + M[1] * G[b][1] + M[2] * G[b][2];
+                    const TT mg1 = M[3] * G[b][0] + M[4] * G[b][1] + M[5] * G[b][2];
+                    const TT mg2 = M[6] * G[b][0] + M[7] * G[b][1] + M[8] * G[b][2];
 #pragma unroll
-                for (int aa = 0; aa < AH; aa++) {
-                    const int ga = half * AH + aa;
-                    acc[aa * NPE + b] += G[ga][0] * mg0 + G[ga][1] * mg1 + G[ga][2] * mg2;
+                    for (int aa = 0; aa < AH; aa++) {
+                        const int ga = half * AH + aa;
+                        // three chained fused multiply-adds into the accumulator
+                        TT t = acc[aa * BH + bb];
+                        t = fma(G[ga][0], mg0, t);
+                        t = fma(G[ga][1], mg1, t);
+                        acc[aa * BH + bb] = fma(G[ga][2], mg2, t);
+                    }
                 }
-            }
+            };
+            if (BSPLIT == 1 || bh == 0)
+                contract(std::integral_constant<int, 0>());
+            else
+                contract(std::integral_constant<int, BH>());
         }
         const TT Vt = (TT)V;
 #pragma unroll
-        for (int r = 0; r < AH * NPE; r++) Ke[9 * (half * AH * NPE + r) + ij] = Vt * acc[r];
+        for (int aa = 0; aa < AH; aa++)
+#pragma unroll
+            for (int bb = 0; bb < BH; bb++)
+                Ke[9 * ((half * AH + aa) * NPE + b0 + bb) + ij] = Vt * acc[aa * BH + bb];
     }
-    if (jj == 0) {
+    if (jj == 0 && bh == 0) {
         double* fe = a.fe + e * (3 * NPE);
 #pragma unroll
         for (int k = 0; k < NPE; k++) {
@@ -932,6 +961,7 @@ void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipS
     if (a.E == 0) return;
     const int nblk = (int)((a.E + kTanElemsPerWave - 1) / kTanElemsPerWave);
 #define CMG_LAUNCH(K) hipLaunchKernelGGL(K, dim3(nblk, a.n_sets), dim3(64), 0, s, a)
+#define CMG_LAUNCH2(K) hipLaunchKernelGGL(K, dim3(nblk, a.n_sets), dim3(128), 0, s, a)
     if (npe == 10) {
         if (variant == 1) {
             if (fp32) CMG_LAUNCH((k_residual_tangent<10, 4, float>));
@@ -939,6 +969,9 @@ void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipS
         } else if (variant == 3) {
             if (fp32) CMG_LAUNCH((k_residual_tangent_v2<10, 4, float, 2>));
             else CMG_LAUNCH((k_residual_tangent_v2<10, 4, double, 2>));
+        } else if (variant == 4) {
+            if (fp32) CMG_LAUNCH2((k_residual_tangent_v2<10, 4, float, 1, 2>));
+            else CMG_LAUNCH2((k_residual_tangent_v2<10, 4, double, 1, 2>));
         } else {
             if (fp32) CMG_LAUNCH((k_residual_tangent_v2<10, 4, float, 1>));
             else CMG_LAUNCH((k_residual_tangent_v2<10, 4, double, 1>));
@@ -953,6 +986,7 @@ void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipS
         }
     }
 #undef CMG_LAUNCH
+#undef CMG_LAUNCH2
 }
 
 void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s) {

@@ -569,8 +569,9 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
     {
         const std::string km = p.Get<std::string>("Solver.GPU.KrylovMethod", "BiCGStab");
         if (km == "GMRES") o.krylov_method = 0;
-        else if (km == "BiCGStab") o.krylov_method = 1;
-        else throw std::runtime_error("Solver.GPU.KrylovMethod must be GMRES or BiCGStab, not " + km);
+        else if (km == "BiCGStabGraph") o.krylov_method = 1;
+        else if (km == "BiCGStab") o.krylov_method = 2;
+        else throw std::runtime_error("Solver.GPU.KrylovMethod must be GMRES, BiCGStab or BiCGStabGraph, not " + km);
     }
     o.transpose_tangent = p.Get<bool>("Solver.GPU.TransposeTangent", false) ? 1 : 0;
     o.pressure_tangent = p.Get<bool>("Solver.GPU.PressureTangent", true) ? 1 : 0;

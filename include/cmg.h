@@ -107,8 +107,11 @@ typedef struct cmg_options {
     int32_t krylov_maxit;       /* Krylov iterations per linear solve (default 4000) */
     int32_t device;             /* HIP device ordinal (default 0) */
     int32_t element_variant;    /* residual+tangent kernel: 2 shared quadrature stage (default),
-                                   3 the same in two node passes, 1 per-lane quadrature points */
-    int32_t krylov_method;      /* 0 GMRES(restart), 1 BiCGStab (default); both right block-Jacobi */
+                                   3 the same in two node passes, 4 the same with two lanes per
+                                   element and spatial pair, 1 per-lane quadrature points */
+    int32_t krylov_method;      /* 0 GMRES(restart), 1 BiCGStab with device-side scalars and graph-replayed
+                                   iterations, 2 BiCGStab with host-side scalars (default); all right
+                                   block-Jacobi preconditioned */
     double newton_atol;         /* Solver.Precision used as SNES abstol/rtol/stol (default 1e-8) */
     double newton_rtol;
     double newton_stol;
@@ -253,7 +256,7 @@ typedef struct cmg_batch_info {
     int32_t use_graph;
     int32_t tuned;             /* 1 once the autotuner has run */
     int32_t reserved[2];
-    double element_ms[3];      /* measured launch time of element variants 1..3 (0: not timed) */
+    double element_ms[4];      /* measured launch time of element variants 1..4 (0: not timed) */
     double spmv_ms[3];         /* measured batched SpMV time at 4, 8, 16 lanes */
     double tune_seconds;
 } cmg_batch_info;

@@ -153,7 +153,8 @@ class CpuMechanics:
                     if np.linalg.norm(rr) <= rn or lam < 1.0 / 64:
                         break
                 except FloatingPointError:
-                    pass
+                    if lam < 2.0 ** -20:     # the step inverts elements however short it is
+                        raise RuntimeError("oracle Newton: every step length inverts an element")
                 lam *= 0.5
             step = np.linalg.norm(dn - d)
             d = dn

@@ -91,6 +91,24 @@ def test_element_tangents_match_complex_step(case):
     assert rel.max() <= 1e-12
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("variant", [1, 3, 4])
+def test_element_kernel_variants_agree_with_the_oracle(variant, precision):
+    """every residual+tangent kernel variant against the complex-step oracle (and the
+    residual against the oracle residual), T10 LV with Holzapfel and active tension"""
+    mesh, mats = lv_small()
+    x = deformed(mesh)
+    with MechanicsContext(mesh, mats, element_variant=variant, precision=precision) as ctx:
+        Kg = ctx.element_tangents(x, 0.12).astype(np.float64)
+        fg = ctx.element_residuals(x, 0.12)
+    rc, Ko, _ = O.tangent_batch(oracle_elements(mesh), x, mats, 0.12, kind="cs")
+    rc2, fo, _ = O.residual_batch(oracle_elements(mesh), x, mats, 0.12)
+    rel = np.linalg.norm((Kg - Ko).reshape(len(Ko), -1), axis=1) / np.linalg.norm(Ko.reshape(len(Ko), -1), axis=1)
+    print(f"variant {variant} precision {precision}: element tangent rel err {rel.max():.2e}")
+    assert rel.max() <= (1e-12 if precision == 0 else 2e-6)
+    assert (np.abs(fg - fo) / np.abs(fo).max(axis=1, keepdims=True)).max() <= 1e-12
+
+
 def test_mixed_precision_element_tangents():
     mesh, mats = lv_small()
     x = deformed(mesh)
@@ -165,7 +183,7 @@ def test_mass_matrix():
     assert np.abs(Mg - Mo).max() <= 1e-14 * np.abs(Mo).max()
 
 
-@pytest.mark.parametrize("method", [0, 1], ids=["gmres", "bicgstab"])
+@pytest.mark.parametrize("method", [0, 1, 2], ids=["gmres", "bicgstab_graph", "bicgstab"])
 def test_spmv_and_krylov_solve(method):
     """The Newmark system matrix K + M/(beta dt^2) at the unloaded state, dt = 1 ms."""
     mesh, mats = lv_small()
@@ -251,7 +269,7 @@ def test_static_mixed_precision_converges_to_fp64_root():
         assert err <= 1e-8
 
 
-@pytest.mark.parametrize("method", [0, 1], ids=["gmres", "bicgstab"])
+@pytest.mark.parametrize("method", [0, 1, 2], ids=["gmres", "bicgstab_graph", "bicgstab"])
 def test_newmark_steps_match_oracle(method):
     """Newmark (beta=1/4, gamma=1/2) with Holzapfel + active tension + cavity pressure."""
     mesh, mats = lv_small()
