@@ -135,7 +135,7 @@ void cmg_default_options(cmg_options* o) {
     o->krylov_maxit = 4000;
     o->krylov_method = 2;   // BiCGStab, host-side scalars: the fastest on the 200k-element LV beat
     o->device = 0;
-    o->element_variant = 2;
+    o->element_variant = 0;   // by precision: 4 (FP64), 2 (mixed), the fastest of each on the LV
     o->newton_atol = 1e-8;
     o->newton_rtol = 1e-8;
     o->newton_stol = 1e-8;
@@ -179,7 +179,8 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
         c->opt = *options;
     else
         cmg_default_options(&c->opt);
-    if (c->opt.element_variant < 1 || c->opt.element_variant > 4) c->opt.element_variant = 2;
+    if (c->opt.element_variant < 1 || c->opt.element_variant > 4)
+        c->opt.element_variant = (c->opt.precision == CMG_PREC_MIXED) ? 2 : 4;
     c->device = c->opt.device;
     CMG_HIP_CHECK(hipSetDevice(c->device));
     CMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

@@ -106,9 +106,10 @@ typedef struct cmg_options {
     int32_t krylov_restart;     /* GMRES restart length (default 80) */
     int32_t krylov_maxit;       /* Krylov iterations per linear solve (default 4000) */
     int32_t device;             /* HIP device ordinal (default 0) */
-    int32_t element_variant;    /* residual+tangent kernel: 2 shared quadrature stage (default),
-                                   3 the same in two node passes, 4 the same with two lanes per
-                                   element and spatial pair, 1 per-lane quadrature points */
+    int32_t element_variant;    /* residual+tangent kernel: 2 shared quadrature stage, 3 the same in
+                                   two node passes, 4 the same with two lanes per element and
+                                   spatial pair, 1 per-lane quadrature points; 0 (default): 4 in
+                                   FP64, 2 in mixed precision */
     int32_t krylov_method;      /* 0 GMRES(restart), 1 BiCGStab with device-side scalars and graph-replayed
                                    iterations, 2 BiCGStab with host-side scalars (default); all right
                                    block-Jacobi preconditioned */
@@ -266,7 +267,7 @@ typedef struct cmg_batch_info {
  * outlive it.  Every set starts from ctx's current node coordinates. */
 int cmg_batch_create(cmg_ctx* ctx, int32_t n_sets, const cmg_material* materials, cmg_batch** out);
 int cmg_batch_destroy(cmg_batch* batch);
-/* use_graph 0/1; spmv_lanes 0 (autotune) or 4/8/16; element_variant 0 (autotune) or 1..3;
+/* use_graph 0/1; spmv_lanes 0 (autotune) or 4/8/16; element_variant 0 (autotune) or 1..4;
  * cg_chunk >= 1.  Call before the first step. */
 int cmg_batch_configure(cmg_batch* batch, int32_t use_graph, int32_t spmv_lanes, int32_t element_variant,
                         int32_t cg_chunk);
