@@ -245,6 +245,23 @@ class MechanicsContext:
             raise CmgError(rc, "cmg_newmark_step")
         return rc, st
 
+    def newmark_solve(self, time, dt, from_trial=False):
+        """Predictor + Newton of a Newmark step, kept pending (see commit_step)."""
+        st = _lib.CmgStepStats()
+        rc = self.L.cmg_newmark_solve(self.h, time, dt, int(bool(from_trial)), ctypes.byref(st))
+        if rc not in (_lib.SUCCESS, _lib.FAILED):
+            raise CmgError(rc, "cmg_newmark_solve")
+        return rc, st
+
+    def trial_volumes(self):
+        """cavity volume per pressure group at the pending step's configuration"""
+        v = np.zeros(self.n_groups)
+        self._check(self.L.cmg_trial_volumes(self.h, _ptr(v)), "cmg_trial_volumes")
+        return v
+
+    def commit_step(self):
+        self._check(self.L.cmg_commit_step(self.h), "cmg_commit_step")
+
 
 class ParameterBatch:
     """S material-parameter sets on the mesh of one `MechanicsContext` (cmg_batch_* in

@@ -41,6 +41,11 @@ double now_s() {
 
 int to_dev_material(const cmg_material& m, DevMaterial& d) {
     if (m.law != CMG_LAW_GUCCIONE && m.law != CMG_LAW_HOLZAPFEL) return fail("unknown material law");
+    if (m.tension_fn < CMG_FN_NONE || m.tension_fn > CMG_FN_TANH) return fail("unknown tension function");
+    if (m.tension_fn == CMG_FN_TANH &&
+        !(m.fn[0] > 0.0 && m.fn[4] >= 0.0 && m.fn[5] > 0.0 && m.fn[7] >= 0.0 && m.fn[8] > 0.0))
+        return fail("TanH tension: cycleLength, ContractionTime and RelaxationTime must be positive, "
+                    "Duration and LengthDependenceUpstrokeTime not negative");
     std::memset(&d, 0, sizeof(d));
     d.law = m.law;
     d.ignore_corrupt = m.ignore_corrupt;
@@ -936,6 +941,26 @@ int krylov(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol
     return gmres(c, vals, b, x, rtol, maxit, st);
 }
 
+// Cavity volume per pressure group at the device coordinates xd: the surface kernel's
+// per-element contributions, summed in surface order on the host like
+// CBCirculationCavity::CalcVolume (CBCirculationCavity.cpp:77-94).
+int group_volumes(cmg_ctx* c, const double* xd, double* volumes) {
+    std::vector<double> vp(c->S);
+    if (c->S > 0) {
+        launch_surface(surf_args(c, xd, true), false, c->stream);
+        CMG_HIP_CHECK(hipMemcpyAsync(vp.data(), c->vol_partial.p, c->S * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+    }
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (c->S > 0 && c->h_sgroup.empty()) {
+        c->h_sgroup.resize(c->S);
+        CMG_HIP_CHECK(hipMemcpy(c->h_sgroup.data(), c->sgroup.p, c->S * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    for (int g = 0; g < c->n_groups; g++) volumes[g] = 0.0;
+    for (int64_t i = 0; i < c->S; i++) volumes[c->h_sgroup[i]] += vp[i];
+    return CMG_SUCCESS;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1058,25 +1083,13 @@ int cmg_surface_forces(cmg_ctx* c, const double* x, const double* pressures, dou
 
 int cmg_cavity_volumes(cmg_ctx* c, const double* x, double* volumes) {
     if (!c || !volumes) return fail("cmg_cavity_volumes: null argument");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
     const double* xd = c->nodes.as<double>();
     if (x) {
         CMG_HIP_CHECK(hipMemcpyAsync(c->xeval.p, x, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
         xd = c->xeval.as<double>();
     }
-    std::vector<double> vp(c->S);
-    if (c->S > 0) {
-        launch_surface(surf_args(c, xd, true), false, c->stream);
-        CMG_HIP_CHECK(hipMemcpyAsync(vp.data(), c->vol_partial.p, c->S * sizeof(double), hipMemcpyDeviceToHost,
-                                     c->stream));
-    }
-    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
-    // summed in surface order on the host: CBCirculationCavity::CalcVolume (:77-94)
-    std::vector<int32_t> grp(c->S, 0);
-    if (c->S > 0)
-        CMG_HIP_CHECK(hipMemcpy(grp.data(), c->sgroup.p, c->S * sizeof(int32_t), hipMemcpyDeviceToHost));
-    for (int g = 0; g < c->n_groups; g++) volumes[g] = 0.0;
-    for (int64_t i = 0; i < c->S; i++) volumes[grp[i]] += vp[i];
-    return CMG_SUCCESS;
+    return group_volumes(c, xd, volumes);
 }
 
 int cmg_mass_matrix(cmg_ctx* c, double* values) {
@@ -1131,6 +1144,7 @@ int cmg_linear_solve(cmg_ctx* c, const double* values, const double* b, double* 
 // ===========================================================================
 int cmg_set_state(cmg_ctx* c, const double* nodes, const double* vel, const double* acc) {
     if (!c) return fail("null context");
+    c->trial_pending = false;
     const size_t vb = c->n * sizeof(double);
     if (nodes) CMG_HIP_CHECK(hipMemcpyAsync(c->nodes.p, nodes, vb, hipMemcpyHostToDevice, c->stream));
     if (vel) CMG_HIP_CHECK(hipMemcpyAsync(c->vel.p, vel, vb, hipMemcpyHostToDevice, c->stream));
@@ -1309,6 +1323,7 @@ extern "C" {
 
 int cmg_static_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
     if (!c) return fail("null context");
+    c->trial_pending = false;
     CMG_HIP_CHECK(hipSetDevice(c->device));
     cmg_step_stats local;
     cmg_step_stats* st = stats ? stats : &local;
@@ -1348,9 +1363,10 @@ int cmg_static_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
     return CMG_SUCCESS;
 }
 
-int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
+int cmg_newmark_solve(cmg_ctx* c, double time, double dt, int32_t guess, cmg_step_stats* stats) {
     if (!c) return fail("null context");
-    if (!(dt > 0.0)) return fail("cmg_newmark_step: dt must be positive");
+    if (!(dt > 0.0)) return fail("cmg_newmark_solve: dt must be positive");
+    if (guess != 0 && guess != 1) return fail("cmg_newmark_solve: guess must be 0 (predictor) or 1 (last trial)");
     CMG_HIP_CHECK(hipSetDevice(c->device));
     cmg_step_stats local;
     cmg_step_stats* st = stats ? stats : &local;
@@ -1358,12 +1374,18 @@ int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) 
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     const double beta = c->opt.newmark_beta, gamma = c->opt.newmark_gamma;
+    // a pending trial of the same step size is a better start than the predictor when
+    // only a load (the cavity pressure) changed since it was solved
+    const bool from_trial = guess == 1 && c->trial_pending && c->trial_dt == dt;
+    c->trial_pending = false;
     // predictor (:845-855): the initial guess is d~
     launch_newmark_predict(n, dt, beta, gamma, c->vel.as<double>(), c->acc.as<double>(), c->dpred.as<double>(),
                            c->vpred.as<double>(), s);
     const double cm = 1.0 / (beta * dt * dt);
     double* d = c->d.as<double>();
-    if (c->has_override && c->override_checked) {
+    if (from_trial) {
+        st->guess_source = 5;   // d still holds the last trial increment
+    } else if (c->has_override && c->override_checked) {
         // proposed guess (learned warm start): kept only if its residual is below the predictor's
         double r_pred, r_prop;
         launch_scale_copy(n, 1.0, c->dpred.as<double>(), d, s);
@@ -1381,14 +1403,36 @@ int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) 
     } else {
         launch_scale_copy(n, 1.0, c->dpred.as<double>(), d, s);
     }
-    c->has_override = false;
-    c->override_checked = false;
+    if (!from_trial) {
+        c->has_override = false;
+        c->override_checked = false;
+    }
     launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
     int rc = newton(c, time, cm, st);
-    if (rc != CMG_SUCCESS) {
-        CMG_HIP_CHECK(hipStreamSynchronize(s));
-        return CMG_FAILED;
-    }
+    CMG_HIP_CHECK(hipStreamSynchronize(s));
+    if (rc != CMG_SUCCESS) return CMG_FAILED;
+    c->trial_pending = true;
+    c->trial_time = time;
+    c->trial_dt = dt;
+    return CMG_SUCCESS;
+}
+
+int cmg_trial_volumes(cmg_ctx* c, double* volumes) {
+    if (!c || !volumes) return fail("cmg_trial_volumes: null argument");
+    if (!c->trial_pending) return fail("cmg_trial_volumes: no solved step is pending");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    launch_scale_copy(c->n, 1.0, c->nodes.as<double>(), c->xeval.as<double>(), c->stream);
+    launch_axpby(c->n, 1.0, c->d.as<double>(), 1.0, c->xeval.as<double>(), c->stream);
+    return group_volumes(c, c->xeval.as<double>(), volumes);
+}
+
+int cmg_commit_step(cmg_ctx* c) {
+    if (!c) return fail("null context");
+    if (!c->trial_pending) return fail("cmg_commit_step: no solved step is pending");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    const double dt = c->trial_dt, beta = c->opt.newmark_beta, gamma = c->opt.newmark_gamma;
     // corrector (:984-1015)
     launch_newmark_correct(n, dt, beta, gamma, c->d.as<double>(), c->dpred.as<double>(), c->vpred.as<double>(),
                            c->vel.as<double>(), c->acc.as<double>(), s);
@@ -1396,8 +1440,15 @@ int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) 
     // increment history (features of a learned warm start)
     launch_scale_copy(n, 1.0, c->inc1.as<double>(), c->inc2.as<double>(), s);
     launch_scale_copy(n, 1.0, c->d.as<double>(), c->inc1.as<double>(), s);
+    c->trial_pending = false;
     CMG_HIP_CHECK(hipStreamSynchronize(s));
     return CMG_SUCCESS;
+}
+
+int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
+    int rc = cmg_newmark_solve(c, time, dt, 0, stats);
+    if (rc != CMG_SUCCESS) return rc;
+    return cmg_commit_step(c);
 }
 
 }  // extern "C"

@@ -77,6 +77,7 @@ struct cmg_ctx {
     std::vector<int32_t> h_colind;
     std::vector<double> h_mval;
     std::vector<double> h_pressure;
+    std::vector<int32_t> h_sgroup;   // surface groups, read back on the first volume query
     int64_t nnzb = 0;
     int64_t n_solid_blocks = 0;
     // element data (SoA)
@@ -100,6 +101,9 @@ struct cmg_ctx {
     bool has_override = false;  // cmg_set_initial_guess / cmg_mlp_propose_guess
     bool override_checked = false;   // compare with the predictor before use (Newmark)
     cmg::DevBuf inc1, inc2;     // increments of the last two Newmark steps
+    // converged Newmark increment not yet committed (cmg_newmark_solve), with its time and step
+    bool trial_pending = false;
+    double trial_time = 0.0, trial_dt = 0.0;
     cmg::DevBuf override_guess;
 };
 

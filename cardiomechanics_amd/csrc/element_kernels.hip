@@ -72,8 +72,25 @@ __device__ double analytic(int type, const double* fn, double time) {
     }
 }
 
-__device__ __forceinline__ double tension_of(const DevMaterial& m, double time, double tact) {
+// Time part of the active tension of an element.  FromFunction: Tmax g(t - t_act)
+// (CBTensionFromFunction.cpp:59-61), the whole tension.  TanH
+// (CBTensionModelTanH.cpp:71-97): t is wrapped into [0, cycleLength] by repeated
+// subtraction, t_s = clamp(t - t_act - t_emd, 0, t_dur) is returned in ts and the
+// value is Tmax S_peak tanh^2((t_dur - t_s)/tau_r); the stretch-dependent factor
+// phi tanh^2(t_s/tau_c(phi)) is applied per quadrature point by eval_law.
+// fn of TanH: cycleLength, l0, t_emd, S_peak, t_dur, tau_c0, ld, ld_up, tau_r, kff,
+// startTime (added to the activation time).
+__device__ __forceinline__ double tension_of(const DevMaterial& m, double time, double tact, double& ts) {
+    ts = 0.0;
     if (m.tension_fn == CMG_FN_NONE) return 0.0;
+    if (m.tension_fn == CMG_FN_TANH) {
+        const double* f = m.fn;
+        double t = time;
+        while (t > f[0]) t -= f[0];
+        ts = fmax(fmin(t - (tact + f[10]) - f[2], f[4]), 0.0);
+        const double tr = tanh((f[4] - ts) / f[8]);
+        return m.tension_max * f[3] * (tr * tr);
+    }
     return m.tension_max * analytic(m.tension_fn, m.fn, time - tact);
 }
 
@@ -142,8 +159,8 @@ struct Law {
 };
 
 // returns false for det F <= 0 (CBStatus::CORRUPT_ELEMENT)
-__device__ __forceinline__ bool eval_law(const DevMaterial& m, const double* F, double Ta, bool want_tangent,
-                                         Law& L) {
+__device__ __forceinline__ bool eval_law(const DevMaterial& m, const double* F, double Ta, double ts,
+                                         bool want_tangent, Law& L) {
 #pragma clang fp contract(off)
     const double J = det3(F);
     if (!m.ignore_corrupt && (J <= 0.0)) return false;
@@ -226,12 +243,32 @@ __device__ __forceinline__ bool eval_law(const DevMaterial& m, const double* F, 
             L.mfs = afs * efs * (1.0 + 2.0 * bfs * I8 * I8);
         }
     }
-    // active stress, FromFunction default coefficients: only T / sqrt(I4f) on ff
+    // active stress with the default stress coefficients (only ff): T / sqrt(I4f)
     if (Ta != 0.0) {
         const double I4f = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
         const double rs = 1. / sqrt(I4f);
-        L.S[0] += Ta * rs;
-        if (want_tangent) L.d00 += -Ta * rs / I4f;
+        if (m.tension_fn == CMG_FN_TANH) {
+            // T(lambda) = Ta phi tanh^2(t_s / tau_c), phi = tanh(ld (lambda - l0)),
+            // tau_c = tau_c0 + ld_up (1 - phi), clipped at 0; S_ff = kff T / lambda
+            const double* f = m.fn;
+            const double lam = sqrt(I4f);
+            const double phi = tanh(f[6] * (lam - f[1]));
+            const double tauc = f[5] + f[7] * (1.0 - phi);
+            const double th = tanh(ts / tauc);
+            const double T = Ta * phi * (th * th);
+            if (T > 0.0) {
+                L.S[0] += f[9] * T * rs;
+                if (want_tangent) {
+                    const double dphi = f[6] * (1.0 - phi * phi);
+                    const double dth = (1.0 - th * th) * (ts / (tauc * tauc)) * (f[7] * dphi);
+                    const double dT = Ta * (dphi * (th * th) + phi * 2.0 * th * dth);   // dT / dlambda
+                    L.d00 += f[9] * (dT * rs * rs - T * rs * rs * rs);
+                }
+            }
+        } else {
+            L.S[0] += Ta * rs;
+            if (want_tangent) L.d00 += -Ta * rs / I4f;
+        }
     }
     return true;
 }
@@ -350,7 +387,8 @@ __global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a0) 
     if (e >= a.E) return;
     const int64_t E = a.E;
     const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
-    const double Ta = tension_of(m, a.time, a.tact[e]);
+    double ts;
+    const double Ta = tension_of(m, a.time, a.tact[e], ts);
     double x[3 * NPE];
 #pragma unroll
     for (int k = 0; k < NPE; k++) {
@@ -370,7 +408,7 @@ __global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a0) 
         for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
         local_F<NPE>(x, g, B, Binv, Floc);
         Law L;
-        if (!eval_law(m, Floc, Ta, false, L)) {
+        if (!eval_law(m, Floc, Ta, ts, false, L)) {
             atomicMin(a.corrupt, (int)e);
             return;
         }
@@ -411,7 +449,8 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
     if (slot >= kTanElemsPerWave || e >= a.E) return;
     const int64_t E = a.E;
     const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
-    const double Ta = tension_of(m, a.time, a.tact[e]);
+    double ts;
+    const double Ta = tension_of(m, a.time, a.tact[e], ts);
     double x[3 * NPE];
 #pragma unroll
     for (int k = 0; k < NPE; k++) {
@@ -436,7 +475,7 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
 #pragma unroll
             for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
             local_F<NPE>(x, g, B, Binv, Floc);
-            if (!eval_law(m, Floc, Ta, true, L)) {
+            if (!eval_law(m, Floc, Ta, ts, true, L)) {
                 atomicMin(a.corrupt, (int)e);
                 return;
             }
@@ -523,7 +562,8 @@ __global__ __launch_bounds__(64 * BSPLIT) void k_residual_tangent_v2(const ElemA
         const int64_t e = e0 + slot;
         if (e < E) {
             const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
-            const double Ta = tension_of(m, a.time, a.tact[e]);
+            double ts;
+    const double Ta = tension_of(m, a.time, a.tact[e], ts);
             double x[3 * NPE];
 #pragma unroll
             for (int k = 0; k < NPE; k++) {
@@ -539,7 +579,7 @@ __global__ __launch_bounds__(64 * BSPLIT) void k_residual_tangent_v2(const ElemA
             for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
             local_F<NPE>(x, g, B, Binv, Floc);
             Law L;
-            if (!eval_law(m, Floc, Ta, true, L)) atomicMin(a.corrupt, (int)e);
+            if (!eval_law(m, Floc, Ta, ts, true, L)) atomicMin(a.corrupt, (int)e);
             {
 #pragma clang fp contract(off)
                 double FS[9], St[9], BinvT[9], Bt[9], P1[9], P[9];

@@ -50,7 +50,9 @@ int main(int argc, char** argv) {
         for (const auto& r : rec) {
             std::printf("t=%.6f newton=%d krylov=%d |r|=%.3e wall=%.3fs", r.time, r.newton, r.krylov, r.residual,
                         r.wall);
-            for (double v : r.volumes) std::printf(" volume=%.9e", v);
+            for (double v : r.volumes) std::printf(" volume=%.17g", v);
+            for (double v : r.pressures) std::printf(" pressure=%.17g", v);
+            if (r.repeats) std::printf(" repeats=%d", r.repeats);
             std::printf("\n");
         }
         std::printf("done: %zu steps in %.3f s\n", rec.size(), wall);

@@ -387,8 +387,30 @@ std::vector<cmg_material> MaterialsFromParameters(const ParameterMap& p, const s
             auto fn = CBAnalyticFunction::Create(ftype, p, own + "FromFunction");
             m.tension_fn = fn->DeviceType();
             fn->DeviceParams(m.fn);
+        } else if (tm == "TanH") {
+            // CBTensionModelTanH (CBTensionModelTanH.cpp:27-125)
+            m.tension_max = p.Get<double>(key("TensionMax"), 0.0);
+            if (m.tension_max != 1000.0)
+                throw std::runtime_error("CBTensionModelTanH: Parameter TensionMax != 1000 is not compatible with "
+                                         "this model to ensure model output in Pa!");
+            auto tkey = [&](const std::string& tail, double dflt) {
+                const std::string o = own + "TanH." + tail, d = def + "TanH." + tail;
+                // the reference reads the fallback key without its tail (InitKey, :18-24): no value
+                if (!p.IsAvailable(o) && p.IsAvailable(d)) return p.Get<double>(def + "TanH");
+                return p.Get<double>(o, dflt);
+            };
+            const char* names[9] = {"cycleLength", "MinFiberStretch", "EMD", "PeakTension", "Duration",
+                                    "ContractionTime", "DegreeOfLengthDependence", "LengthDependenceUpstrokeTime",
+                                    "RelaxationTime"};
+            const double dflts[9] = {1.0, 0.7, 0.015, 120.0, 0.3, 0.1, 5.0, 0.5, 0.1};
+            for (int k = 0; k < 9; k++) m.fn[k] = tkey(names[k], dflts[k]);
+            m.fn[9] = tkey("kff", 1.0);
+            m.fn[10] = tkey("startTime", 0.0);
+            if (tkey("kss", 0.0) != 0.0 || tkey("knn", 0.0) != 0.0 || tkey("ksn", 0.0) != 0.0)
+                throw std::runtime_error("TanH stress coefficients kss/knn/ksn are not on this path (only kff)");
+            m.tension_fn = CMG_FN_TANH;
         } else if (tm != "None") {
-            throw std::runtime_error("tension model " + tm + " is not on this path (FromFunction)");
+            throw std::runtime_error("tension model " + tm + " is not on this path (FromFunction, TanH)");
         }
         out.push_back(m);
     }
@@ -544,6 +566,10 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
     std::set<int> wanted;
     for (const auto& gk : p.GetChildNodes("Plugins.ApplyPressureFromFunction.Groups"))
         for (int sid : p.GetArray<int>(gk + ".Surfaces")) wanted.insert(sid);
+    const bool circulation = p.Get<bool>("Solver.Plugins.Circulation", false);
+    if (circulation)
+        for (const auto& ck : p.GetChildNodes("Plugins.Circulation.Circs"))
+            if (p.Get<bool>(ck + ".Active", false)) wanted.insert(p.Get<int>(ck + ".Cavities.VentrSurface"));
     groupIds_.assign(wanted.begin(), wanted.end());
     const int64_t S = (int64_t)m.surfaceIndex.size();
     for (int64_t s = 0; s < S; s++) {
@@ -553,6 +579,10 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
         sel_group.push_back((int32_t)(it - groupIds_.begin()));
     }
     pressure_.Init(p, groupIds_);
+    if (circulation) {
+        if (type_ != "NewmarkBeta") throw std::runtime_error("the circulation plugin needs Solver.Type = NewmarkBeta");
+        circulation_.Init(p, groupIds_);
+    }
     cmg_options o;
     cmg_default_options(&o);
     const double prec = p.Get<double>("Solver.Precision", 1e-8);
@@ -597,6 +627,29 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
     check(cmg_sizes(ctx_, &nn, &ne, &nDofs_, &nnzb_), "cmg_sizes");
     formulation_ = std::make_unique<CBFormulationTotalLagrangianGPU>(this);
     exportFile_ = p.Get<std::string>("Plugins.ApplyPressureFromFunction.ExportFile", "");
+    appliedPressures_.assign(std::max<size_t>(1, groupIds_.size()), 0.0);
+    if (circulation_.Active()) circulation_.InitVolumes(CavityVolumes(), timing_.GetTimeStep());
+}
+
+// Apply of every plugin before a solve (CBSolverNewmarkBeta.cpp:839-840): the
+// prescribed pressures, then the circulation's estimates for its cavities.
+void CBSolverGPU::ApplyPlugins(double time) {
+    if (!pressure_.Active() && !circulation_.Active()) return;
+    const double dt = timing_.GetTimeStep();
+    std::fill(appliedPressures_.begin(), appliedPressures_.end(), 0.0);
+    if (pressure_.Active()) {
+        pressure_.Apply(time, dt);
+        appliedPressures_ = pressure_.Pressures();
+    }
+    if (circulation_.Active()) {
+        circulation_.Apply(time, dt);
+        for (const auto& gp : circulation_.Pressures()) appliedPressures_[gp.first] = gp.second;
+        if (circulation_.TakeZeroVelocityRequest()) {
+            std::vector<double> z(nDofs_, 0.0);
+            check(cmg_set_state(ctx_, nullptr, z.data(), z.data()), "cmg_set_state");
+        }
+    }
+    check(cmg_set_pressures(ctx_, appliedPressures_.data()), "cmg_set_pressures");
 }
 
 const std::vector<double>& CBSolverGPU::Nodes() {
@@ -612,17 +665,35 @@ std::vector<double> CBSolverGPU::CavityVolumes() {
 }
 
 CBStatus CBSolverGPU::SolverStep(double time) {
+    return SolverStep(time, false, nullptr);
+}
+
+// CBSolverNewmarkBeta::SolverStep / CBSolverEquilibrium::SolverStep with the plugins:
+// with the circulation the converged trial is analysed before it is accepted
+// (CBSolverNewmarkBeta.cpp:928-976); REPEAT leaves it pending for the next solve.
+CBStatus CBSolverGPU::SolverStep(double time, bool repeat, cmg_step_stats* stats) {
     currentTime_ = time;
-    if (pressure_.Active()) {
-        pressure_.Apply(time, timing_.GetTimeStep());
-        check(cmg_set_pressures(ctx_, pressure_.Pressures().data()), "cmg_set_pressures");
-    }
-    cmg_step_stats st;
-    const int rc = (type_ == "Static") ? cmg_static_step(ctx_, time, timing_.GetTimeStep(), &st)
-                                       : cmg_newmark_step(ctx_, time, timing_.GetTimeStep(), &st);
-    if (rc == CMG_SUCCESS) return CBStatus::SUCCESS;
+    cmg_step_stats local;
+    cmg_step_stats* st = stats ? stats : &local;
+    std::memset(st, 0, sizeof(*st));
+    ApplyPlugins(time);
+    const double dt = timing_.GetTimeStep();
+    int rc;
+    if (type_ == "Static") rc = cmg_static_step(ctx_, time, dt, st);
+    else if (!circulation_.Active()) rc = cmg_newmark_step(ctx_, time, dt, st);
+    else rc = cmg_newmark_solve(ctx_, time, dt, repeat ? 1 : 0, st);
+    if (rc != CMG_SUCCESS && rc != CMG_FAILED)
+        throw std::runtime_error(std::string("solver step failed: ") + cmg_last_error());
     if (rc == CMG_FAILED) return CBStatus::FAILED;
-    throw std::runtime_error(std::string("solver step failed: ") + cmg_last_error());
+    if (circulation_.Active()) {
+        std::vector<double> vol(appliedPressures_.size());
+        check(cmg_trial_volumes(ctx_, vol.data()), "cmg_trial_volumes");
+        circulation_.AnalyzeResults(vol);
+        const CBStatus f = circulation_.GetStatus();
+        if (f == CBStatus::FAILED || f == CBStatus::REPEAT) return f;
+        check(cmg_commit_step(ctx_), "cmg_commit_step");
+    }
+    return CBStatus::SUCCESS;
 }
 
 // CBSolver::Run (CBSolver.cpp:1724-1858): first pass at t = StartTime, then steps until StopTime
@@ -640,6 +711,8 @@ void CBSolverGPU::Run(std::vector<StepRecord>* records, int maxSteps) {
     int steps = 0;
     timing_.ResetForSim();
     while (timing_.IsValid(time) || first) {
+        bool repeat = false;
+        int repeats = 0;
         while (true) {
             double t = time + timing_.GetTimeStep();
             if (first) {
@@ -647,29 +720,26 @@ void CBSolverGPU::Run(std::vector<StepRecord>* records, int maxSteps) {
                 t = timing_.GetStartTime();
             }
             cmg_step_stats st;
-            std::memset(&st, 0, sizeof(st));
             const auto w0 = std::chrono::steady_clock::now();
-            currentTime_ = t;
-            if (pressure_.Active()) {
-                pressure_.Apply(t, timing_.GetTimeStep());
-                check(cmg_set_pressures(ctx_, pressure_.Pressures().data()), "cmg_set_pressures");
-            }
-            const int rc = (type_ == "Static") ? cmg_static_step(ctx_, t, timing_.GetTimeStep(), &st)
-                                               : cmg_newmark_step(ctx_, t, timing_.GetTimeStep(), &st);
-            if (rc != CMG_SUCCESS && rc != CMG_FAILED)
-                throw std::runtime_error(std::string("solver step failed: ") + cmg_last_error());
+            const CBStatus rc = SolverStep(t, repeat, &st);
             const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
-            timing_.SetSolverStatus(rc == CMG_SUCCESS ? CBStatus::SUCCESS : CBStatus::FAILED);
+            // plugins not asking for the repeat go back to their last accepted state
+            if (rc == CBStatus::FAILED || (rc == CBStatus::REPEAT && circulation_.GetStatus() != CBStatus::REPEAT))
+                circulation_.StepBack();
+            if (rc == CBStatus::REPEAT) repeats++;
+            repeat = rc == CBStatus::REPEAT;
+            timing_.SetSolverStatus(rc);
             const CBStatus next = timing_.ComputeNextTimeStep();
-            if (rc == CMG_SUCCESS) {
+            if (rc == CBStatus::SUCCESS) {
                 std::vector<double> vol = CavityVolumes();
-                if (records) records->push_back({t, first ? 0.0 : t - time, st.newton_iterations,
-                                                 st.krylov_iterations, st.residual_norm, wall, vol});
+                if (records)
+                    records->push_back({t, first ? 0.0 : t - time, st.newton_iterations, st.krylov_iterations,
+                                        st.residual_norm, wall, vol, repeats, appliedPressures_});
                 if (out) {
                     // pressure in mmHg and volume in ml (CBApplyPressureFromFunction.cpp:167)
                     out << std::setprecision(7) << std::fixed << std::setw(16) << t;
                     for (size_t g = 0; g < groupIds_.size(); g++)
-                        out << std::setw(16) << pressure_.Pressures()[g] / 133.322 << std::setw(16) << vol[g] * 1e6;
+                        out << std::setw(16) << appliedPressures_[g] / 133.322 << std::setw(16) << vol[g] * 1e6;
                     out << "\n";
                 }
                 time = t;
@@ -682,7 +752,10 @@ void CBSolverGPU::Run(std::vector<StepRecord>* records, int maxSteps) {
             first = false;
             timing_.ResetCounter();
         }
+        circulation_.WriteToFile(time);
+        const bool exitNow = circulation_.ExitCheck(time);
         if (maxSteps > 0 && ++steps >= maxSteps) break;
+        if (exitNow) break;
     }
 }
 

@@ -8,6 +8,8 @@
 //   CBFormulationTotalLagrangianGPU   CalcNodalForces / CalcNodalForcesJacobian (CBFormulation.h:30-41)
 //   CBTiming                          adaptive time step (CBSolver/CBTiming.cpp:25-162)
 //   CBApplyPressureFromFunction       prescribed cavity pressure (CBApplyPressureFromFunction.cpp:20-132)
+//   CBCirculation, CBCircWindkessel3  cavity pressure from a 0D circulation (CBCirculation.cpp,
+//                                     CBCircWindkessel3.cpp, CBCircModel.hpp)
 //   CBSolverGPU                       Init / SolverStep / Run (CBSolver.h:68-215) with the static
 //                                     (CBSolverEquilibrium) and Newmark (CBSolverNewmarkBeta) steps
 #pragma once
@@ -156,6 +158,115 @@ private:
 };
 
 // ---------------------------------------------------------------------------
+// 0D circulation (mechanics/src/CBCavity/CBCircModel.h, CBCircWindkessel3.cpp) and the
+// plugin that couples it to the cavity volumes (CBSolverPlugins/CBCirculation.cpp).
+// Pressures are in Plugins.Circulation.PressureUnit, volumes in VolumeUnit.
+class CBCircModelBase {
+public:
+    CBCircModelBase(const ParameterMap& p, const std::string& prefix, int nC, int nS, int nR);
+    virtual ~CBCircModelBase() = default;
+    int GetNumberOfCavities() const { return nC_; }
+    const std::vector<int>& CavitySurfaceIndices() const { return cavitySurfaces_; }
+    const std::vector<double>& PreloadingPressures() const { return preloading_; }   // [2 nC]
+    virtual void SetInitialCavityVolumes(const double* v) = 0;
+    virtual void GetEstCavityVolumes(const double* p, double dt, double* v) = 0;
+    void AcceptEstState();
+    void RevertPrevState();
+    bool SteadyStateCheck(double time);
+    void WriteHeaderToFile();
+    void WriteResultsToFile(double time);
+    double Result(const std::string& name) const;
+    const std::vector<double>& State() const { return state_; }
+
+protected:
+    void Integrate(const double* p, double dt);   // RK4 at constant cavity pressures (CBCircModel.hpp:187-236)
+    virtual void Algebraics(const double* p, const double* u) = 0;
+    virtual void Rates(const double* p, const double* u, double* du) = 0;
+    std::string name_, filename_;
+    int nC_, nS_, nR_;
+    double maxTimeStep_;
+    std::vector<double> state_, prevState_, estState_;
+    std::vector<std::string> stateNames_, resultNames_;
+    std::vector<double> results_;
+    std::vector<double> cavityPressures_, prevCavityPressures_, estCavityPressures_;
+    std::vector<int> cavitySurfaces_;
+    std::vector<double> preloading_;
+    bool steadyActive_ = false;
+    double steadyStart_ = 0, steadyPeriod_ = 0, steadyThresh_ = 0;
+    std::string steadyParam_;
+    double prevCheckValue_ = 0, prevCheckTime_ = 0, prevCheckTimeActive_ = 0;
+    void InitSteadyStateCheck(const ParameterMap& p, const std::string& prefix);
+};
+
+class CBCircWindkessel3 : public CBCircModelBase {
+public:
+    CBCircWindkessel3(const ParameterMap& p, const std::string& prefix);
+    void SetInitialCavityVolumes(const double* v) override { state_[0] = v[0]; }
+    void GetEstCavityVolumes(const double* p, double dt, double* v) override;
+
+private:
+    void Algebraics(const double* p, const double* u) override;
+    void Rates(const double* p, const double* u, double* du) override;
+    double artResist_, artCompli_, perResist_, venResist_, venPressure_, initialArtVolume_;
+    bool resetArtVolume_;
+};
+
+class CBCirculation {
+public:
+    // groupSurfaceIds: surface index of each pressure group of the context
+    void Init(const ParameterMap& p, const std::vector<int32_t>& groupSurfaceIds);
+    // InitCircModels' volume part: FE cavity volumes (m^3 per pressure group) at the start
+    void InitVolumes(const std::vector<double>& volumes, double dt);
+    void Apply(double time, double dt);
+    // cavity pressure in Pa per pressure group this plugin drives
+    std::map<int, double> Pressures() const;
+    void AnalyzeResults(const std::vector<double>& volumes);
+    void StepBack();
+    CBStatus GetStatus() const { return status_; }
+    void WriteToFile(double time);
+    bool ExitCheck(double time);
+    bool TakeZeroVelocityRequest() { bool z = zeroVelocity_; zeroVelocity_ = false; return z; }
+    bool Active() const { return !models_.empty(); }
+    const std::vector<int>& CavitySurfaces() const { return surfaces_; }
+    const std::vector<double>& EstimatedPressures() const { return est_; }
+    std::vector<double> CouplingResidual() const;
+    const std::vector<std::unique_ptr<CBCircModelBase>>& Models() const { return models_; }
+
+private:
+    static constexpr int N_ = 5;   // past pressures kept for the extrapolation
+    void GetCircVolume(const std::vector<double>& p, std::vector<double>& v);
+    std::vector<double> Extrapolate() const;
+    void ExtrapolatePressure();
+    void IncreasePressure();
+    void PerturbPressure(int i);
+    void AssembleComplianceMatrices(int i);
+    void EstimatePressureUsingSecant();
+    void EstimatePressureUsingQuasiNewton();
+    void EstimatePressureUsingNewton();
+    std::vector<std::unique_ptr<CBCircModelBase>> models_;
+    std::vector<int> surfaces_, groups_;
+    int nC_ = 0;
+    CBStatus status_ = CBStatus::DACCORD;
+    double uV_ = 1e6, uP_ = 133.322, maxP_ = 0, deltaP_ = 0, dP_ = 0;
+    double tolerance_ = 1e-4, startTime_ = 0, preloadingTime1_ = 0, preloadingTime2_ = 0;
+    double dampingFactor_ = 0, dampingDeclineFactor_ = 0;
+    bool dampingActive_ = false, dampingIteration_ = false, perturbation_ = true;
+    int maxIterations_ = 100, nNonNewton_ = 10, nSecant_ = 3;
+    bool hasStarted_ = false, stepBack_ = false, preloadingFinished_ = false;
+    bool lastPreloadingStep1_ = false, lastPreloadingStep2_ = false, zeroVelocity_ = false;
+    double dt_ = 0, prevDt_ = 0;
+    double timeStep_[N_] = {0, 0, 0, 0, 0};
+    std::vector<std::vector<double>> pressure_;
+    std::vector<double> est_, prevEst_, fine_, prevFine_, circ_, prevCirc_, residual_, prevResidual_;
+    std::vector<double> preload1_, preload2_;
+    std::vector<double> fineCompli_, circCompli_, sysElast_;   // nC x nC row-major
+    unsigned long n_ = 1;
+    unsigned i_ = 0, j_ = 0;
+    unsigned long iSum_ = 0;
+    std::string itsFile_;
+};
+
+// ---------------------------------------------------------------------------
 class CBSolverGPU;
 
 class CBFormulationTotalLagrangianGPU {
@@ -174,6 +285,8 @@ struct StepRecord {
     int newton, krylov;
     double residual, wall;
     std::vector<double> volumes;
+    int repeats = 0;                 // plugin-requested repeats (circulation coupling iterations - 1)
+    std::vector<double> pressures;   // cavity pressures applied (Pa per pressure group)
 };
 
 class CBSolverGPU {
@@ -182,6 +295,8 @@ public:
     // Solver.Type = Static | NewmarkBeta; materials from Materials.Mat_<id>.*
     void Init(const ParameterMap& p, TetgenModel& model);
     CBStatus SolverStep(double time);
+    // repeat: the previous call of this step answered REPEAT (start from its trial increment)
+    CBStatus SolverStep(double time, bool repeat, cmg_step_stats* stats);
     void Run(std::vector<StepRecord>* records = nullptr, int maxSteps = -1);
     cmg_ctx* Context() { return ctx_; }
     const std::vector<double>& Nodes();   // current coordinates
@@ -189,6 +304,7 @@ public:
     std::vector<double>& JacobianValues() { return jacobian_; }
     double CurrentTime() const { return currentTime_; }
     CBTiming& GetTiming() { return timing_; }
+    CBCirculation& GetCirculation() { return circulation_; }
     std::vector<double> CavityVolumes();
     const std::string& Type() const { return type_; }
 
@@ -198,6 +314,9 @@ private:
     cmg_ctx* ctx_ = nullptr;
     CBTiming timing_;
     CBApplyPressureFromFunction pressure_;
+    CBCirculation circulation_;
+    std::vector<double> appliedPressures_;
+    void ApplyPlugins(double time);
     std::unique_ptr<CBFormulationTotalLagrangianGPU> formulation_;
     std::vector<double> nodes_, forces_, jacobian_;
     std::vector<int32_t> groupIds_;

@@ -5,6 +5,7 @@ Mirrors how the reference reads them (key names kept so settings round-trip):
   Holzapfel  CBConstitutiveModelHolzapfel::Init  (CBConstitutiveModelHolzapfel.cpp:18-85)
   density    CBMaterial::Init                    (CBMaterial.cpp:19-74)
   tension    CBTensionFromFunction               (CBTensionFromFunction.cpp:36-56)
+             CBTensionModelTanH                  (CBTensionModelTanH.cpp:27-125)
   functions  CBAnalyticFunction subclasses       (CBData/CBAnalyticFunction.h:39-176)
 Per-material keys fall back to Materials.Mat_Default.<key> when absent.
 """
@@ -13,9 +14,12 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass, field
 
+from .settings import as_bool
+
 LAW_GUCCIONE = 0
 LAW_HOLZAPFEL = 1
 FN_NONE, FN_LINEAR, FN_SINUS, FN_SINUSDRIVER, FN_DOUBLEHILL = 0, 1, 2, 3, 4
+FN_TANH = 5   # the TanH tension model (a stretch-dependent law, not a time function)
 _FN_NAMES = {"None": FN_NONE, "Linear": FN_LINEAR, "Sinus": FN_SINUS, "SinusDriver": FN_SINUSDRIVER,
              "DoubleHill": FN_DOUBLEHILL}
 
@@ -42,6 +46,31 @@ def guccione(C, bf, bt, bfs, K, density=1082.0, a_scale=1.0, b_scale=1.0, **kw) 
 
 def holzapfel(a, b, af, bf, as_, bs, afs, bfs, kappa, k=100.0, density=1082.0, **kw) -> Material:
     return Material(law=LAW_HOLZAPFEL, params=[a, b, af, bf, as_, bs, afs, bfs, kappa, k], density=density, **kw)
+
+
+# TanH keys and defaults in the order of the device parameter block fn[0..8]
+# (CBTensionModelTanH.cpp:99-118)
+TANH_KEYS = (("cycleLength", 1.0), ("MinFiberStretch", 0.7), ("EMD", 0.015), ("PeakTension", 120.0),
+             ("Duration", 0.3), ("ContractionTime", 0.1), ("DegreeOfLengthDependence", 5.0),
+             ("LengthDependenceUpstrokeTime", 0.5), ("RelaxationTime", 0.1))
+
+
+def tanh_params(kff=1.0, start_time=0.0, **kw):
+    """fn block of the TanH tension model: the nine TANH_KEYS (keyword = key name),
+    kff and startTime.  Tmax must be 1000 (the model's output is in kPa)."""
+    vals = dict(TANH_KEYS)
+    for k, v in kw.items():
+        if k not in vals:
+            raise KeyError(f"unknown TanH parameter {k}")
+        vals[k] = float(v)
+    return FN_TANH, [vals[k] for k, _ in TANH_KEYS] + [float(kff), float(start_time), 0.0]
+
+
+def with_tanh(m: Material, **kw) -> Material:
+    """the material with the TanH tension model (Tmax = 1000, TanH parameters as keywords)"""
+    m.tension_max = 1000.0
+    m.tension_fn, m.fn = tanh_params(**kw)
+    return m
 
 
 def function_params(kind: str, start=0.0, stop=0.0, keep_max=True, omega=0.0, phi=0.0,
@@ -110,7 +139,7 @@ def from_parameters(params: dict, mat_index) -> Material:
     """Build a Material from dotted reference keys (Appendix of the settings files)."""
     law = str(_get(params, mat_index, "Type", "Guccione"))
     density = float(_get(params, mat_index, "Density", 0.0))
-    ignore = int(bool(_get(params, mat_index, "IgnoreCorruptElements", False)))
+    ignore = int(as_bool(_get(params, mat_index, "IgnoreCorruptElements", False)))
     if law == "Guccione":
         m = guccione(*(float(_get(params, mat_index, f"Guccione.{k}")) for k in ("C", "bf", "bt", "bfs", "K")),
                      density=density, a_scale=float(_get(params, mat_index, "Guccione.aScale", 1.0)),
@@ -123,7 +152,26 @@ def from_parameters(params: dict, mat_index) -> Material:
     else:
         raise ValueError(f"material law {law} is not on this path (Guccione, Holzapfel)")
     tmodel = str(_get(params, mat_index, "TensionModel", "None"))
-    if tmodel == "FromFunction":
+    if tmodel == "TanH":
+        tmax = float(_get(params, mat_index, "TensionMax", 0.0))
+        if tmax != 1000.0:     # CBTensionModelTanH.cpp:32-35
+            raise ValueError("CBTensionModelTanH: TensionMax != 1000 is not compatible with this model")
+        own, dflt = f"Materials.Mat_{mat_index}.TanH", "Materials.Mat_Default.TanH"
+
+        def key(tail, d):
+            if own + "." + tail not in params and dflt + "." + tail in params:
+                # the reference reads the fallback key without its tail (InitKey, :18-24),
+                # which is no value: its Get throws
+                raise KeyError(dflt)
+            return float(params.get(own + "." + tail, d))
+
+        for k in ("kss", "knn", "ksn"):
+            if key(k, 0.0) != 0.0:
+                raise ValueError("TanH stress coefficients kss/knn/ksn are not on this path (only kff)")
+        vals = {k: key(k, d) for k, d in TANH_KEYS}
+        m.tension_max = tmax
+        m.tension_fn, m.fn = tanh_params(kff=key("kff", 1.0), start_time=key("startTime", 0.0), **vals)
+    elif tmodel == "FromFunction":
         # Tmax has no Mat_Default fallback (CBTensionFromFunction.cpp:44)
         m.tension_max = float(params[f"Materials.Mat_{mat_index}.TensionMax"])
         base = f"Materials.Mat_{mat_index}.FromFunction"
@@ -131,11 +179,13 @@ def from_parameters(params: dict, mat_index) -> Material:
         g = lambda k, d: float(params.get(f"{base}.{k}", d))  # noqa: E731
         m.tension_fn, m.fn = function_params(
             kind, start=g("StartTime", 0.0), stop=g("StopTime", 0.0),
-            keep_max=bool(params.get(f"{base}.KeepMaxValue", True)), omega=g("Sinus.Omega", 0.0),
+            keep_max=as_bool(params.get(f"{base}.KeepMaxValue", True)), omega=g("Sinus.Omega", 0.0),
             phi=g("Sinus.Phi", 0.0), period=g("DoubleHill.Period", 0.8),
             contr_offset=g("DoubleHill.ContrTimeOffset", 0.215), relax_offset=g("DoubleHill.RelaxTimeOffset", 0.362),
             contr_rate=g("DoubleHill.ContrRateConst", 1.32), relax_rate=g("DoubleHill.RelaxRateConst", 21.9),
             onset=g("DoubleHill.OnsetTime", 0.0))
+    elif tmodel != "None":
+        raise ValueError(f"tension model {tmodel} is not on this path (FromFunction, TanH)")
     return m
 
 

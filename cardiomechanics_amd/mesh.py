@@ -270,7 +270,7 @@ def lv_dims_for(n_target: int):
 
 
 def lv_mesh(n_target: int = 200_000, dims=None, order: int = 10, surface_type: int = 3,
-            fibre_deg=(60.0, -60.0), activation=(0.0, 0.08)) -> Mesh:
+            fibre_deg=(60.0, -60.0), activation=(0.0, 0.08), endo_normal: str = "cavity") -> Mesh:
     """Idealised truncated prolate-ellipsoid LV (Land et al. 2015 benchmark shape).
 
     The wall is a structured (transmural t, meridian s, circumferential v) grid:
@@ -286,7 +286,12 @@ def lv_mesh(n_target: int = 200_000, dims=None, order: int = 10, surface_type: i
     benchmark Problem2.xml) and the cavity volume sum_i x_0.(a x b)/6 is negative.  Fibres: helix angle linear in t from
     fibre_deg[0] (endocardium) to fibre_deg[1] (epicardium), sheet = transmural.
     Activation rises linearly apex -> base over `activation` seconds.
+    endo_normal="wall" turns the endocardium the other way (normal out of the blood
+    volume, into the wall): positive pressure inflates and the cavity volume is
+    positive, the orientation a circulation-coupled cavity needs.
     """
+    if endo_normal not in ("cavity", "wall"):
+        raise ValueError("endo_normal must be 'cavity' or 'wall'")
     nt, ns, nv = dims if dims is not None else lv_dims_for(n_target)
     t = np.linspace(0.0, 1.0, nt + 1)
     rs = LV_ENDO[0] + (LV_EPI[0] - LV_ENDO[0]) * t
@@ -327,6 +332,8 @@ def lv_mesh(n_target: int = 200_000, dims=None, order: int = 10, surface_type: i
     faces = orient_faces_outward(coords, boundary_faces(tets), tets)
     is_endo = np.all(tcoord_v[faces] == 0.0, axis=1)
     endo = faces[is_endo]                                      # normal out of the wall, into the cavity
+    if endo_normal == "wall":
+        endo = endo[:, [0, 2, 1]]
     nvtx = coords.shape[0]
     if order == 10:
         coords, el, edges = t4_to_t10(coords, tets)

@@ -6,7 +6,9 @@
                             per surface group, piecewise intervals of an analytic function,
                             pressure = Offset + Amplitude * g(t - Offset) on [start, stop + dt/2)
   run                       CBSolver::Run (CBSolver.cpp:1724-1858): a first pass at t = start,
-                            then successful steps until StopTime; REPEAT retries, FAILED halves dt
+                            then successful steps until StopTime; REPEAT retries, FAILED halves dt;
+                            plugins that analyse trial steps (circulation.Circulation) decide
+                            whether a converged Newmark trial is accepted
 The solver steps themselves (static Newton, Newmark) run on the GPU in libcmg.so.
 """
 from __future__ import annotations
@@ -109,6 +111,9 @@ class ApplyPressureFromFunction:
             p[g] = v
         return p
 
+    def pressures(self):
+        return dict(self.pressure)
+
 
 def linear_ramp(amplitude, start=0.0, stop=1.0, group=0):
     fn = [start, stop, 1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0]
@@ -132,41 +137,91 @@ class StepRecord:
     residual: float
     wall: float
     volumes: np.ndarray
+    repeats: int = 0          # plugin-requested repeats of this step (circulation coupling iterations - 1)
 
 
-def run(ctx, kind: str, timing: Timing, plugin: ApplyPressureFromFunction | None = None,
-        on_step=None, warm_start=None, max_steps=None):
-    """CBSolver::Run with the GPU solver step.  kind = 'static' or 'newmark'.
-    warm_start(ctx, t, dt) may return an increment used as the Newton initial guess."""
+def run(ctx, kind: str, timing: Timing, plugin=None, on_step=None, warm_start=None, max_steps=None,
+        plugins=None, repeat_from_trial=True):
+    """CBSolver::Run (CBSolver.cpp:1724-1858) with the GPU solver step.  kind = 'static'
+    or 'newmark'.  plugin / plugins: objects with apply(time, dt) and pressures()
+    ({pressure group: Pa}); a plugin with analyze(volumes) (the circulation) sees the
+    cavity volumes of each converged Newmark trial before it is accepted
+    (CBSolverNewmarkBeta.cpp:928-976) and answers through its `status`: REPEAT solves
+    the step again with the plugin's next pressure estimate (from the last trial's
+    increment when repeat_from_trial, else from the predictor as the reference),
+    FAILED rejects it; on a rejected step every plugin not asking for a repeat steps
+    back.  warm_start(ctx, t, dt) may return an increment used as the Newton initial
+    guess.  Stops early when every plugin's exit_check says so."""
+    plugins = list(plugins or []) + ([plugin] if plugin is not None else [])
+    analyzers = [p for p in plugins if hasattr(p, "analyze")]
+    if analyzers and kind != "newmark":
+        raise ValueError("plugins that analyse trial steps need the Newmark solver")
     records = []
     time = timing.start
     first = True
     timing.reset_for_init()
     steps = 0
+    if analyzers:
+        vols0 = ctx.cavity_volumes()
+        for p in analyzers:
+            if hasattr(p, "init_volumes"):
+                p.init_volumes(vols0, timing.dt)
     while timing.is_valid(time) or first:
+        repeat = False
+        repeats = 0
         while True:
             t = time + timing.dt
             if first:
                 time = timing.start - timing.dt
                 t = timing.start
-            p = plugin.apply(t, timing.dt) if plugin else {}
-            if plugin:
-                ctx.set_pressures(plugin.values(ctx.n_groups))
-            if warm_start is not None:
+            if plugins:
+                pr = np.zeros(ctx.n_groups)
+                for p in plugins:
+                    p.apply(t, timing.dt)
+                    for g, v in p.pressures().items():
+                        pr[g] = v
+                    if getattr(p, "wants_zero_velocity", False):
+                        ctx.set_state(velocity=np.zeros(ctx.n_dofs), acceleration=np.zeros(ctx.n_dofs))
+                        p.wants_zero_velocity = False
+                ctx.set_pressures(pr)
+            if warm_start is not None and not repeat:
                 g = warm_start(ctx, t, timing.dt)
                 if g is not None:
                     ctx.set_initial_guess(g)
             t0 = _time.perf_counter()
             if kind == "static":
                 rc, st = ctx.static_step(t, timing.dt)
-            else:
+            elif not analyzers:
                 rc, st = ctx.newmark_step(t, timing.dt)
+            else:
+                rc, st = ctx.newmark_solve(t, timing.dt, from_trial=repeat and repeat_from_trial)
+                if rc == _lib.SUCCESS:
+                    vols = ctx.trial_volumes()
+                    for p in analyzers:
+                        p.analyze(vols)
+                    fb = [p.status for p in analyzers]
+                    if _lib.FAILED in fb:
+                        rc = _lib.FAILED
+                    elif _lib.REPEAT in fb:
+                        rc = _lib.REPEAT
+                    else:
+                        ctx.commit_step()
             wall = _time.perf_counter() - t0
+            if rc == _lib.FAILED:
+                for p in plugins:
+                    if hasattr(p, "step_back"):
+                        p.step_back()
+            elif rc == _lib.REPEAT:
+                repeats += 1
+                for p in plugins:
+                    if hasattr(p, "step_back") and getattr(p, "status", None) != _lib.REPEAT:
+                        p.step_back()
+            repeat = rc == _lib.REPEAT
             ok = timing.compute_next(rc)
             if rc == _lib.SUCCESS:
                 vols = ctx.cavity_volumes()
                 rec = StepRecord(t, timing.dt if first else t - time, rc, st.newton_iterations,
-                                 st.krylov_iterations, st.residual_norm, wall, vols)
+                                 st.krylov_iterations, st.residual_norm, wall, vols, repeats)
                 records.append(rec)
                 if on_step:
                     on_step(rec, st)
@@ -178,7 +233,13 @@ def run(ctx, kind: str, timing: Timing, plugin: ApplyPressureFromFunction | None
             first = False
             timing.reset_for_sim()
             timing.reset_counter()
+        for p in plugins:
+            if hasattr(p, "write"):
+                p.write(time)
+        exits = [p.exit_check(time) for p in plugins if hasattr(p, "exit_check")]
         steps += 1
+        if exits and any(exits):
+            break
         if max_steps is not None and steps >= max_steps:
             break
     return records

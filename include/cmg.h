@@ -48,6 +48,7 @@ extern "C" {
 #define CMG_FN_SINUS 2
 #define CMG_FN_SINUSDRIVER 3
 #define CMG_FN_DOUBLEHILL 4
+#define CMG_FN_TANH 5   /* the TanH tension model (CBTensionModelTanH.cpp), not a time function */
 
 #define CMG_PREC_FP64 0
 #define CMG_PREC_MIXED 1
@@ -61,7 +62,15 @@ typedef struct cmg_ctx cmg_ctx;
  *   Holzapfel params: a, b, af, bf, as, bs, afs, bfs, kappa, k
  *   fn: startTime, stopTime, keepMaxValue, omega, phi, DoubleHill period,
  *       contrTimeOffset, relaxTimeOffset, contrRateConst, relaxRateConst,
- *       onsetTime, driverMax                                                  */
+ *       onsetTime, driverMax
+ * tension_fn = CMG_FN_TANH selects the TanH tension model instead
+ * (CBTensionModelTanH.cpp:71-97): T = Tmax max(S_peak phi tanh^2(t_s/tau_c)
+ * tanh^2((t_dur - t_s)/tau_r), 0) with phi = tanh(ld (lambda_f - l0)),
+ * tau_c = tau_c0 + ld_up (1 - phi), t_s = clamp(t - t_act - startTime - t_emd, 0, t_dur)
+ * (t wrapped into one cycle), active PK2 kff T / lambda_f on ff;
+ *   fn: cycleLength, MinFiberStretch l0, EMD t_emd, PeakTension S_peak, Duration t_dur,
+ *       ContractionTime tau_c0, DegreeOfLengthDependence ld,
+ *       LengthDependenceUpstrokeTime ld_up, RelaxationTime tau_r, kff, startTime     */
 typedef struct cmg_material {
     int32_t law;
     int32_t ignore_corrupt;   /* Materials.Mat_<id>.IgnoreCorruptElements */
@@ -145,7 +154,8 @@ typedef struct cmg_step_stats {
                                     1 cmg_set_initial_guess, 2 proposed guess kept (its residual was
                                     below the predictor's), 3 proposed guess rejected, 4 the
                                     initial guess inverted an element and Newton restarted from a
-                                    zero increment */
+                                    zero increment, 5 the pending trial increment
+                                    (cmg_newmark_solve with guess 1) */
     int32_t reserved_s;
     double guess_residual_ratio; /* ||r(proposed)|| / ||r(predictor)|| when a guess was proposed */
 } cmg_step_stats;
@@ -234,6 +244,19 @@ int cmg_static_step(cmg_ctx* ctx, double time, double dt, cmg_step_stats* stats)
 /* CBSolverNewmarkBeta::SolverStep (CBSolverNewmarkBeta.cpp:829-1019): predictor,
  * Newton on M(d - d~)/(beta dt^2) + f_int + f_p = 0, corrector. */
 int cmg_newmark_step(cmg_ctx* ctx, double time, double dt, cmg_step_stats* stats);
+/* The same step in two halves, for plugins that analyse the converged state before it
+ * is accepted (CBSolverNewmarkBeta.cpp:928-976; CBCirculation::AnalyzeResults,
+ * CBCirculation.cpp): cmg_newmark_solve runs the predictor and Newton and keeps the
+ * converged increment pending without changing nodes, velocity or acceleration;
+ * cmg_trial_volumes gives the cavity volume per pressure group at nodes + increment;
+ * cmg_commit_step applies the corrector and nodes += increment.  A plugin's REPEAT is
+ * another cmg_newmark_solve of the same step (nothing to undo).  guess: 0 the
+ * predictor (as the reference), 1 the pending trial increment when its dt equals
+ * this dt (else the predictor; stats->guess_source = 5 when it was used).
+ * cmg_newmark_step = cmg_newmark_solve(guess 0) + cmg_commit_step. */
+int cmg_newmark_solve(cmg_ctx* ctx, double time, double dt, int32_t guess, cmg_step_stats* stats);
+int cmg_trial_volumes(cmg_ctx* ctx, double* volumes /* [n_groups] */);
+int cmg_commit_step(cmg_ctx* ctx);
 
 /* ---- Parameter-set batch ----------------------------------------------------
  * No counterpart in the reference, which solves one parameter set per run: S

@@ -72,6 +72,8 @@ def lib():
             "orc_energy_dF_cs": (ctypes.c_int32, [_M, _D, _D]),
             "orc_pk2_dF_cs": (ctypes.c_int32, [_M, _D, _D]),
             "orc_active_stress": (None, [_D, ctypes.c_double, _D]),
+            "orc_tanh_tension": (ctypes.c_double, [_M, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "orc_active_stress_model": (None, [_M, _D, ctypes.c_double, ctypes.c_double, _D]),
             "orc_t10_shape": (None, [_D, _D, _D]),
             "orc_t10_residual": (ctypes.c_int32, [_D, _D, _D, ctypes.c_double, _U8, _M, ctypes.c_double,
                                                   ctypes.c_double, _D]),
@@ -189,6 +191,19 @@ def pk2_dF_cs(mat, F):
     out = np.zeros(81)
     rc = lib().orc_pk2_dF_cs(ctypes.byref(m), pF, out.ctypes.data_as(_D))
     return rc, out.reshape(9, 9)
+
+
+def tanh_tension(mat, lam, time, tact=0.0):
+    m = material(mat)
+    return lib().orc_tanh_tension(ctypes.byref(m), float(lam), float(time), float(tact))
+
+
+def active_stress_model(mat, F, time, tact=0.0):
+    m = material(mat)
+    F, pF = _d(F)
+    out = np.zeros(9)
+    lib().orc_active_stress_model(ctypes.byref(m), pF, float(time), float(tact), out.ctypes.data_as(_D))
+    return out.reshape(3, 3)
 
 
 def active_stress(F, tension):

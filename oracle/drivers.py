@@ -176,15 +176,35 @@ class CpuMechanics:
         self.nodes = self.nodes + self.d
         return it, rn
 
-    def newmark_step(self, time, dt, beta=0.25, gamma=0.5, tol=1e-12):
+    def newmark_solve(self, time, dt, beta=0.25, gamma=0.5, tol=1e-12, from_trial=False):
+        """predictor + Newton (CBSolverNewmarkBeta.cpp:845-923); the increment stays
+        pending until commit_step"""
         dpred = dt * self.vel + dt * dt * 0.5 * (1 - 2 * beta) * self.acc
         vpred = 1.0 * self.vel + (1 - gamma) * dt * self.acc
         cm = 1.0 / (beta * dt * dt)
-        d, it, rn = self.newton(time, dpred.copy(), cm=cm, dpred=dpred, tol=tol)
-        self.acc = 1.0 / (beta * dt * dt) * d + -1.0 / (beta * dt * dt) * dpred
-        self.vel = vpred + gamma * dt * self.acc
-        self.d = d
-        self.nodes = self.nodes + d
+        pend = getattr(self, "_pending", None)
+        d0 = pend["d"].copy() if (from_trial and pend is not None and pend["dt"] == dt) else dpred.copy()
+        self._pending = None
+        d, it, rn = self.newton(time, d0, cm=cm, dpred=dpred, tol=tol)
+        self._pending = dict(d=d, dpred=dpred, vpred=vpred, dt=dt, beta=beta, gamma=gamma)
+        return it, rn
+
+    def trial_volumes(self):
+        return self.cavity_volumes(self.nodes + self._pending["d"])
+
+    def commit_step(self):
+        """corrector (CBSolverNewmarkBeta.cpp:978-1015)"""
+        p = self._pending
+        dt, beta, gamma = p["dt"], p["beta"], p["gamma"]
+        self.acc = 1.0 / (beta * dt * dt) * p["d"] + -1.0 / (beta * dt * dt) * p["dpred"]
+        self.vel = p["vpred"] + gamma * dt * self.acc
+        self.d = p["d"]
+        self.nodes = self.nodes + p["d"]
+        self._pending = None
+
+    def newmark_step(self, time, dt, beta=0.25, gamma=0.5, tol=1e-12):
+        it, rn = self.newmark_solve(time, dt, beta, gamma, tol)
+        self.commit_step()
         return it, rn
 
     def cavity_volumes(self, x=None):

@@ -148,7 +148,7 @@ double analytic(int type, const double* fn, double time, double prev) {
 // Tmax * g(t - t_act).  Linear is evaluated with the previous time equal to
 // the current one (the steady value within a step).
 double tension(const orc_material* m, double time, double tact) {
-    if (m->tension_fn == ORC_FN_NONE) return 0.0;
+    if (m->tension_fn == ORC_FN_NONE || m->tension_fn == ORC_FN_TANH) return 0.0;
     double tt = time - tact;
     return m->tension_max * analytic(m->tension_fn, m->fn, tt, tt);
 }
@@ -280,6 +280,42 @@ template <class T> void active(const M3<T>& F, double Ta, M3<T>& Sa) {
     Sa.e[0] = Ta * (1. / std::sqrt(I4f));
 }
 
+// CBTensionModelTanH::CalcActiveTension (CBTensionModelTanH.cpp:71-97), the
+// reference's statement order: lambda = |column 0 of the local F|,
+// phi = tanh(ld (lambda - l0)), tau_c = tau_c0 + ld_up (1 - phi), t wrapped into
+// one cycle by repeated subtraction, t_s = max(min(t - t_act - t_emd, t_dur), 0),
+// Ta = max(S_peak phi tanh(t_s/tau_c)^2 tanh((t_dur - t_s)/tau_r)^2, 0), T = Tmax Ta.
+// The activation time includes startTime (AddToActivationTime, :90).
+template <class T> T tanh_tension(const orc_material* m, const T& lambda, double time, double tact) {
+    const double* f = m->fn;
+    const double l0 = f[1], t_emd = f[2], S_peak = f[3], t_dur = f[4], tau_c0 = f[5], ld = f[6], ld_up = f[7];
+    const double tau_r = f[8];
+    T phi = std::tanh(ld * (lambda - l0));
+    T tau_c = tau_c0 + ld_up * (1.0 - phi);
+    double t = time;
+    while (t > f[0]) t -= f[0];
+    const double t_s = std::max(std::min(t - (tact + f[10]) - t_emd, t_dur), 0.0);
+    T a = std::tanh(t_s / tau_c);
+    const double b = std::tanh((t_dur - t_s) / tau_r);
+    T Ta = S_peak * phi * (a * a) * (b * b);
+    if (std::real(Ta) < 0.0) Ta = T(0.0);
+    return m->tension_max * Ta;
+}
+
+// CalcActiveStress (CBTensionModel.h:37-50) of the element's tension model; TanH with
+// its default stress coefficients scaled by kff (kss = knn = ksn = 0).
+template <class T>
+void active_model(const orc_material* m, const M3<T>& F, double time, double tact, M3<T>& Sa) {
+    if (m->tension_fn != ORC_FN_TANH) {
+        active(F, tension(m, time, tact), Sa);
+        return;
+    }
+    Sa = zero3<T>();
+    T I4f = F.e[0] * F.e[0] + F.e[3] * F.e[3] + F.e[6] * F.e[6];
+    T lam = std::sqrt(I4f);
+    Sa.e[0] = m->fn[9] * tanh_tension(m, lam, time, tact) * (1. / lam);
+}
+
 // ---------------------------------------------------------------------------
 // T10 element, CBElement/CBElementSolidT10.cpp
 // ---------------------------------------------------------------------------
@@ -359,14 +395,13 @@ int t10_residual(const T* x, const double* dNdX, const double* bases, double det
         BinvT[n] = from<T>(Bi.e);
         Fq[n] = mul(mul(tr(Bq[n]), Fq[n]), BinvT[n]);
     }
-    const double Ta = tension(m, time, tact);
     M3<T> P[4];
     for (int q = 0; q < 4; q++) {
         M3<T> S;
         int rc = pk2(m, Fq[q], S);
         if (rc != ORC_SUCCESS) return rc;
         M3<T> Sa;
-        active(Fq[q], Ta, Sa);
+        active_model(m, Fq[q], time, tact, Sa);
         for (int i = 0; i < 9; i++) S.e[i] += Sa.e[i];
         S = tr(mul(Fq[q], S));
         P[q] = mul(mul(BinvT[q], S), tr(Bq[q]));
@@ -453,7 +488,7 @@ int t4_residual(const T* x, const double* dNdX, const double* basis, double vol,
     int rc = pk2(m, F, S);
     if (rc != ORC_SUCCESS) return rc;
     M3<T> Sa;
-    active(F, tension(m, time, tact), Sa);
+    active_model(m, F, time, tact, Sa);
     for (int i = 0; i < 9; i++) S.e[i] += Sa.e[i];
     S = mul(S, tr(F));
     S = mul(mul(BinvT, S), tr(B));
@@ -606,6 +641,16 @@ int32_t orc_pk2_dF_cs(const orc_material* m, const double* F, double* dSdF) {
         for (int a = 0; a < 9; a++) dSdF[9 * a + b] = S.e[a].imag() / kCsStep;
     }
     return ORC_SUCCESS;
+}
+
+double orc_tanh_tension(const orc_material* m, double lambda, double time, double tact) {
+    return tanh_tension<double>(m, lambda, time, tact);
+}
+
+void orc_active_stress_model(const orc_material* m, const double* F, double time, double tact, double* Sa) {
+    M3<double> S;
+    active_model<double>(m, from<double>(F), time, tact, S);
+    std::memcpy(Sa, S.e, sizeof(S.e));
 }
 
 void orc_active_stress(const double* F, double tension_value, double* Sa) {

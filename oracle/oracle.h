@@ -36,7 +36,9 @@ enum {
     ORC_FN_LINEAR = 1,
     ORC_FN_SINUS = 2,
     ORC_FN_SINUSDRIVER = 3,
-    ORC_FN_DOUBLEHILL = 4
+    ORC_FN_DOUBLEHILL = 4,
+    ORC_FN_TANH = 5          /* TanH tension model; fn = cycleLength, l0, t_emd, S_peak, t_dur,
+                                tau_c0, ld, ld_up, tau_r, kff, startTime */
 };
 
 /* CBStatus values (mechanics/src/CBStatus/CBStatus.h:20-37). */
@@ -73,6 +75,11 @@ int32_t orc_energy_dF_cs(const orc_material* m, const double* F, double* dWdF);
 /* complex-step dS/dF: out[9*a + b] = dS_a / dF_b */
 int32_t orc_pk2_dF_cs(const orc_material* m, const double* F, double* dSdF);
 void orc_active_stress(const double* F, double tension, double* Sa);
+/* CBTensionModelTanH::CalcActiveTension (CBTensionModelTanH.cpp:71-97): tension at
+ * fibre stretch lambda, time t, element activation time tact */
+double orc_tanh_tension(const orc_material* m, double lambda, double time, double tact);
+/* active PK2 of the element's tension model at the local F (FromFunction or TanH) */
+void orc_active_stress_model(const orc_material* m, const double* F, double time, double tact, double* Sa);
 
 /* ---- T10 (CBElementSolidT10.cpp) ------------------------------------- */
 void orc_t10_shape(const double* X30, double* dNdX150, double* detJ);

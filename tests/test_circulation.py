@@ -1,0 +1,302 @@
+"""Circulation coupling (host logic, CPU): the Windkessel-3 model's RK4 integration
+against closed-form solutions of its linear phases, the pressure extrapolation and
+the Broyden update against the properties that define them, and the whole coupling
+loop of `solver.run` on a linear stand-in for the mechanics whose coupled solution
+is computed independently by bracketed root finding."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.optimize import brentq
+
+from cardiomechanics_amd import _lib, circulation, solver
+
+MMHG = 1.33322e2
+
+
+def wk3_params(**over):
+    p = {
+        "Plugins.Circulation.Tolerance": 1e-7,
+        "Plugins.Circulation.MaxIterations": 100,
+        "Plugins.Circulation.PressureUnit": "mmHg",
+        "Plugins.Circulation.VolumeUnit": "ml",
+        "Plugins.Circulation.Circs.Circ_1.Active": True,
+        "Plugins.Circulation.Circs.Circ_1.Type": "CircWindkessel3",
+        "Plugins.Circulation.Circs.Circ_1.MaxIntegrationTimeStep": 1e-4,
+        "Plugins.Circulation.Circs.Circ_1.CircParameters.ArtResist": 0.5,
+        "Plugins.Circulation.Circs.Circ_1.CircParameters.ArtCompli": 0.05,
+        "Plugins.Circulation.Circs.Circ_1.CircParameters.PerResist": 20.0,
+        "Plugins.Circulation.Circs.Circ_1.CircParameters.VenResist": 0.05,
+        "Plugins.Circulation.Circs.Circ_1.CircParameters.VenPressure": 2.0,
+        "Plugins.Circulation.Circs.Circ_1.InitialConditions.ArtVolume": 0.5,
+        "Plugins.Circulation.Circs.Circ_1.Cavities.VentrSurface": 1,
+    }
+    p.update(over)
+    return p
+
+
+def wk3(**over):
+    return circulation.CircWindkessel3(wk3_params(**over), "Plugins.Circulation.Circs.Circ_1")
+
+
+# ---- the model --------------------------------------------------------------------
+
+def test_wk3_isovolumic_decay_matches_rk4_factor_and_exponential():
+    """Both valves closed (2 < p < V_art/C): V_ven is constant and V_art' = -V_art/(C Rp),
+    so each RK4 sub-step multiplies V_art by 1 - z + z^2/2 - z^3/6 + z^4/24, z = h/(C Rp)."""
+    m = wk3()
+    m.set_initial_cavity_volumes([2.0])
+    p = np.array([5.0])                        # 2 mmHg < 5 < 0.5 / 0.05 = 10 mmHg
+    dt = 3.7e-3
+    m.integrate(p, dt)
+    n = math.ceil(dt / 1e-4)
+    z = (dt / n) / (0.05 * 20.0)
+    factor = 1 - z + z * z / 2 - z ** 3 / 6 + z ** 4 / 24
+    assert m.est_state[0] == 2.0
+    assert m.est_state[1] == pytest.approx(0.5 * factor ** n, rel=1e-14)
+    assert m.est_state[1] == pytest.approx(0.5 * math.exp(-dt / 1.0), rel=1e-12)
+    assert m.results["ArtFlow"] == 0.0 and m.results["VenFlow"] == 0.0
+    assert m.results["PerFlow"] == pytest.approx(m.est_state[1] / 0.05 / 20.0)
+
+
+def test_wk3_ejection_closed_form():
+    """Aortic valve open at constant p: V_art relaxes exponentially to p C Rp/(Ra + Rp)
+    with tau = C Ra Rp/(Ra + Rp), and the ventricle loses exactly the aortic inflow."""
+    Ra, C, Rp = 0.5, 0.05, 20.0
+    m = wk3()
+    m.set_initial_cavity_volumes([2.0])
+    p = 40.0
+    dt = 0.02
+    m.integrate(np.array([p]), dt)
+    vinf = p * C * Rp / (Ra + Rp)
+    tau = C * Ra * Rp / (Ra + Rp)
+    va = vinf + (0.5 - vinf) * math.exp(-dt / tau)
+    # int_0^dt (p - V_art/C)/Ra dt
+    outflow = (p / Ra) * dt - (vinf * dt + (0.5 - vinf) * tau * (1 - math.exp(-dt / tau))) / (C * Ra)
+    assert m.est_state[1] == pytest.approx(va, rel=1e-9)
+    assert m.est_state[0] == pytest.approx(2.0 - outflow, rel=1e-9)
+    assert m.results["ArtPressure"] == p
+    assert m.results["VenFlow"] == 0.0
+
+
+def test_wk3_filling_closed_form():
+    """Mitral valve open (p < VenPressure), aortic closed: V_ven grows linearly at
+    (Pven - p)/Rven."""
+    m = wk3()
+    m.set_initial_cavity_volumes([1.0])
+    m.integrate(np.array([0.5]), 0.01)
+    assert m.est_state[0] == pytest.approx(1.0 + 0.01 * (2.0 - 0.5) / 0.05, rel=1e-13)
+
+
+def test_accept_and_revert():
+    m = wk3()
+    m.set_initial_cavity_volumes([2.0])
+    s0 = m.state.copy()
+    m.integrate(np.array([30.0]), 0.01)
+    est = m.est_state.copy()
+    m.accept_est_state()
+    assert np.array_equal(m.state, est) and np.array_equal(m.prev_state, s0)
+    assert m.cavity_pressures[0] == 30.0
+    m.revert_prev_state()
+    assert np.array_equal(m.state, s0)
+
+
+# ---- plugin pieces ---------------------------------------------------------------------
+
+def _plugin(**over):
+    return circulation.Circulation(wk3_params(**over), groups={1: 0})
+
+
+def test_extrapolation_linear_exact_and_quadratic_error():
+    """Exact for a linear pressure history whatever the step sizes; for p = a t^2 on a
+    uniform grid the difference-quotient form falls short of p(t + dt) by a dt^2."""
+    c = _plugin()
+    c.n = 7
+    dts = [1e-3, 2e-3, 0.5e-3, 1.5e-3, 1e-3]
+    t = np.cumsum([0.0] + dts)           # times of the five stored pressures
+    for k in range(5):                   # pressure[(n - 5 + k) % N] at t[k]
+        c.pressure[(c.n - 5 + k) % c.N] = np.array([3.0 + 7.0 * t[k]])
+        if k:
+            c.time_step[(c.n - 5 + k) % c.N] = dts[k - 1]
+    c.dt = 0.8e-3
+    assert c._extrapolate()[0] == pytest.approx(3.0 + 7.0 * (t[4] + c.dt), rel=1e-13)
+    h = 1e-3
+    for k in range(5):
+        c.pressure[(c.n - 5 + k) % c.N] = np.array([2.0 * (k * h) ** 2])
+        c.time_step[(c.n - 5 + k) % c.N] = h
+    c.dt = h
+    tn = 4 * h
+    assert c._extrapolate()[0] == pytest.approx(2.0 * (tn + h) ** 2 - 2.0 * h * h, rel=1e-12)
+
+
+def test_broyden_update_satisfies_secant_condition():
+    c = circulation.Circulation(wk3_params(), groups={1: 0})
+    rng = np.random.default_rng(3)
+    c.nC = 3
+    c.sys_elast = np.eye(3) + 0.1 * rng.standard_normal((3, 3))
+    c.residual = rng.standard_normal(3)
+    c.fine, c.circ = rng.standard_normal(3), rng.standard_normal(3)
+    c.est, c.prev_est = rng.standard_normal(3), rng.standard_normal(3)
+    c.i, c.n = 2, 20
+    dF = (c.fine - c.circ) - c.residual
+    dX = c.est - c.prev_est
+    est0 = c.est.copy()
+    c._quasi_newton()
+    assert np.allclose(c.sys_elast @ dF, dX, rtol=1e-12, atol=1e-12)
+    assert np.allclose(c.est, est0 - c.sys_elast @ (c.fine - c.circ))
+
+
+def test_unknown_settings_raise():
+    with pytest.raises(ValueError):
+        circulation.Circulation(wk3_params(**{"Plugins.Circulation.PressureUnit": "bar"}), groups={1: 0})
+    with pytest.raises(ValueError):
+        circulation.Circulation(wk3_params(), groups={2: 0})
+    with pytest.raises(ValueError):
+        circulation.Circulation(wk3_params(**{"Plugins.Circulation.Circs.Circ_1.Type": "CircWholeHeart"}),
+                                groups={1: 0})
+
+
+# ---- the coupling loop on a linear stand-in for the mechanics -----------------------------
+
+class LinearCavity:
+    """Stand-in with the context's step interface: the cavity volume is
+    V0(t) + C p (m^3, p in Pa); V0 shrinks during an active contraction."""
+    n_groups = 1
+    n_dofs = 3
+
+    def __init__(self, compliance_ml_per_mmhg=0.1, v_rest_ml=2.5, squeeze_ml=2.0, twitch=0.3):
+        self.C = compliance_ml_per_mmhg * 1e-6 / MMHG
+        self.v_rest = v_rest_ml * 1e-6
+        self.squeeze = squeeze_ml * 1e-6
+        self.twitch = twitch
+        self.p = 0.0
+        self.t_committed = 0.0
+        self.pending = None
+        self.solves = 0
+
+    def v0(self, t):
+        a = math.sin(math.pi * t / self.twitch) ** 2 if 0.0 <= t <= self.twitch else 0.0
+        return self.v_rest - self.squeeze * a
+
+    def volume(self, t, p):
+        return self.v0(t) + self.C * p
+
+    def set_pressures(self, p):
+        self.p = float(np.asarray(p)[0])
+
+    def set_state(self, **kw):
+        pass
+
+    def cavity_volumes(self):
+        return np.array([self.volume(self.t_committed, self.p_committed if hasattr(self, "p_committed") else 0.0)])
+
+    def newmark_solve(self, t, dt, from_trial=False):
+        self.solves += 1
+        self.pending = (t, self.p)
+        return _lib.SUCCESS, _lib.CmgStepStats()
+
+    def trial_volumes(self):
+        t, p = self.pending
+        return np.array([self.volume(t, p)])
+
+    def commit_step(self):
+        self.t_committed, self.p_committed = self.pending
+        self.pending = None
+
+
+def coupled_reference(mech, params, times, dt):
+    """p_n with V_FE(t_n, p_n) = V_circ(state_{n-1}, p_n) by bracketed root finding, the
+    state then advanced with p_n (the plugin's fixed point, found independently)."""
+    m = circulation.CircWindkessel3(params, "Plugins.Circulation.Circs.Circ_1")
+    # the circulation starts from the FE volume of the last preloading step (t_0 - dt, p = 0)
+    m.set_initial_cavity_volumes([mech.volume(times[0] - dt, 0.0) * 1e6])
+    out = []
+    for t in times:
+        def f(p):
+            m.integrate(np.array([p]), dt)
+            return mech.volume(t, p * MMHG) * 1e6 - m.est_state[0]
+        p = brentq(f, -50.0, 500.0, xtol=1e-13, rtol=1e-14)
+        f(p)
+        m.accept_est_state()
+        out.append(p)
+    return np.array(out)
+
+
+def _run_coupled(params, stop=0.4, dt=2e-3):
+    mech = LinearCavity()
+    plug = circulation.Circulation(params, groups={1: 0})
+    timing = solver.Timing(start=0.0, stop=stop, dt=dt)
+    residual = []
+    recs = solver.run(mech, "newmark", timing, plugins=[plug],
+                      on_step=lambda rec, st: residual.append(plug.coupling_residual()[0]))
+    last = {}
+    for (t, i, kind, est) in plug.log:
+        last[round(t / dt)] = est[0]          # the last estimate of a step is the accepted one
+    times = np.array([r.time for r in recs[2:]])
+    accepted = np.array([last[round(t / dt)] for t in times])
+    return mech, plug, recs, times, accepted, np.array(residual[2:])
+
+
+@pytest.mark.parametrize("mode", [
+    {},                                                               # secant, perturbation, Newton, Broyden
+    {"Plugins.Circulation.Perturbation": False, "Plugins.Circulation.SecantIterations": 0},
+], ids=["perturbation", "quasi_newton"])
+def test_coupling_matches_independent_fixed_point(mode):
+    params = wk3_params(**mode)
+    dt = 2e-3
+    mech, plug, recs, times, got, res = _run_coupled(params, dt=dt)
+    # the first pass (t = 0) and the step after it are the two preloading phases
+    assert len(recs) == 201 and recs[1].time == pytest.approx(dt)
+    ref = coupled_reference(mech, params, times, dt)
+    assert np.all(res <= 1e-7)
+    # |V_FE - V_circ| <= 1e-7 ml and |dV/dp| >= C_fe = 0.1 ml/mmHg  =>  |dp| <= 1e-6 mmHg
+    assert np.max(np.abs(got - ref)) < 2e-6, np.max(np.abs(got - ref))
+    assert ref.max() > 12.0 and ref.min() < 2.0     # ejection (V_art/C = 10 mmHg) and filling happened
+    assert max(r.repeats for r in recs) < 40
+
+
+def test_coupling_damping_declines_and_then_converges():
+    """Damped steps accept a blend of the previous and the converged pressure (their
+    residual is not checked); once the factor has declined below 1e-5 every accepted
+    step meets the tolerance again."""
+    params = wk3_params(**{"Plugins.Circulation.CouplingDamping.InitialFactor": 0.5,
+                           "Plugins.Circulation.CouplingDamping.DeclineFactor": 0.8})
+    mech, plug, recs, times, got, res = _run_coupled(params)
+    assert not plug.damping_active
+    kinds = [k for (_, _, k, _) in plug.log]
+    n_damped = kinds.count("damping")
+    assert n_damped == math.ceil(math.log(1e-5 / 0.5) / math.log(0.8))
+    last_damped = max(i for i, k in enumerate(kinds) if k == "damping")
+    t_last = plug.log[last_damped][0]
+    assert np.all(res[times > t_last + 1e-9] <= 1e-7)
+
+
+def test_host_cpp_circulation_matches_python(tmp_path):
+    """The host C++ CBCirculation (cm_circulation.cpp) and the Python Circulation take
+    the same estimates on the linear cavity: accepted pressures and repeat counts agree
+    step for step."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    host = os.path.join(root, "cardiomechanics_amd", "host")
+    exe = tmp_path / "circ_linear"
+    pkg = os.path.dirname(host)
+    if not os.path.exists(os.path.join(pkg, "libcmg.so")):
+        pytest.skip("libcmg.so not built")
+    # links the host layer (and through it libcmg.so); no device call is made
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-I", os.path.join(root, "include"), "-I", host,
+                           os.path.join(root, "tests", "host", "circ_linear.cpp"),
+                           os.path.join(host, "cm_circulation.cpp"), os.path.join(host, "cm_host.cpp"),
+                           "-L", pkg, "-lcmg", f"-Wl,-rpath,{pkg}", "-o", str(exe)])
+    for mode in ({}, {"Plugins.Circulation.Perturbation": False, "Plugins.Circulation.SecantIterations": 0}):
+        params = wk3_params(**mode)
+        dt = 2e-3
+        mech = LinearCavity()
+        args = [str(exe), repr(dt), "0.4", repr(mech.C), repr(mech.v_rest), repr(mech.squeeze), repr(mech.twitch)]
+        args += [f"{k}={str(v).lower() if isinstance(v, bool) else v}" for k, v in params.items()]
+        out = subprocess.run(args, capture_output=True, text=True, check=True).stdout.split("\n")
+        rows = np.array([[float(x) for x in ln.split()] for ln in out if ln.strip()])
+        _, plug, recs, times, got, res = _run_coupled(params, dt=dt)
+        assert rows.shape[0] == len(recs)
+        np.testing.assert_allclose(rows[2:, 1], got, rtol=1e-10, atol=1e-10)   # summation order differs
+        assert [int(r) for r in rows[:, 2]] == [r.repeats for r in recs]

@@ -1,0 +1,138 @@
+"""Windkessel-3 circulation coupled to the LV beat on the GPU (cmg_newmark_solve /
+cmg_trial_volumes / cmg_commit_step through the C ABI).
+
+Every accepted step of the coupled GPU run is re-solved by the CPU oracle from the
+GPU's state before the step at the accepted cavity pressure, starting from the GPU's
+increment (mechanics parity per step: the GPU's root is the oracle's root), and its
+finite-element volume meets the circulation's volume within the coupling tolerance
+(coupling parity).  The split step itself is checked against
+cmg_newmark_step bit for bit."""
+import numpy as np
+import pytest
+
+from oracle.drivers import CpuMechanics
+from cardiomechanics_amd import circulation, solver
+from cardiomechanics_amd import materials as MT
+from cardiomechanics_amd import mesh as M
+from cardiomechanics_amd.context import CmgError, MechanicsContext
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15)
+
+
+def lv_wall_normals():
+    return M.lv_mesh(dims=(2, 4, 8), endo_normal="wall"), [MT.lv_holzapfel(tension_max=120e3)]
+
+
+def wk3_params(tol=1e-7):
+    pre = "Plugins.Circulation.Circs.Circ_1"
+    return {
+        "Plugins.Circulation.Tolerance": tol,
+        "Plugins.Circulation.MaxIterations": 100,
+        "Plugins.Circulation.PressureUnit": "mmHg",
+        "Plugins.Circulation.VolumeUnit": "ml",
+        pre + ".Active": True,
+        pre + ".Type": "CircWindkessel3",
+        pre + ".CircParameters.ArtResist": 2.0,
+        pre + ".CircParameters.ArtCompli": 0.01,
+        pre + ".CircParameters.PerResist": 100.0,
+        pre + ".CircParameters.VenResist": 1.0,
+        pre + ".CircParameters.VenPressure": 1.0,
+        pre + ".InitialConditions.ArtVolume": 0.02,     # 2 mmHg: the aortic valve opens early in the twitch
+        pre + ".Cavities.VentrSurface": 1,
+    }
+
+
+def test_split_step_equals_newmark_step():
+    mesh, mats = lv_wall_normals()
+    dt = 0.005
+    with MechanicsContext(mesh, mats, **TIGHT) as a, MechanicsContext(mesh, mats, **TIGHT) as b:
+        with pytest.raises(CmgError):
+            a.commit_step()                 # nothing pending
+        with pytest.raises(CmgError):
+            a.trial_volumes()
+        for k in range(1, 4):
+            t = k * dt
+            a.set_pressures([300.0 * k])
+            b.set_pressures([300.0 * k])
+            rc, _ = a.newmark_step(t, dt)
+            assert rc == 2
+            rc, st = b.newmark_solve(t, dt)
+            assert rc == 2 and st.guess_source == 0
+            before = b.get_state()
+            vt = b.trial_volumes()
+            host = b.cavity_volumes(before["nodes"] + before["increment"])
+            assert abs(vt[0] - host[0]) <= 1e-14 * abs(host[0])
+            b.commit_step()
+            sa, sb = a.get_state(), b.get_state()
+            for key in ("nodes", "velocity", "acceleration"):
+                assert np.array_equal(sa[key], sb[key]), key
+        # a repeat of the same step from the pending trial converges to the same root
+        b.set_pressures([1000.0])
+        rc, st1 = b.newmark_solve(4 * dt, dt)
+        x1 = b.get_state()["increment"]
+        b.set_pressures([1010.0])
+        rc, st2 = b.newmark_solve(4 * dt, dt, from_trial=True)
+        assert rc == 2 and st2.guess_source == 5
+        b.set_pressures([1000.0])
+        rc, st3 = b.newmark_solve(4 * dt, dt, from_trial=True)
+        x3 = b.get_state()["increment"]
+        assert np.linalg.norm(x3 - x1) <= 1e-9 * np.linalg.norm(x1)
+        assert st3.newton_iterations <= st1.newton_iterations
+
+
+def test_windkessel_coupled_beat_matches_oracle_per_step():
+    mesh, mats = lv_wall_normals()
+    params = wk3_params()
+    dt = 0.005
+    snaps = []
+    with MechanicsContext(mesh, mats, **TIGHT) as ctx:
+        groups = {int(s): k for k, s in enumerate(ctx.group_ids)}
+        plug = circulation.Circulation(params, groups=groups)
+        snaps.append(ctx.get_state())
+
+        def on_step(rec, st):
+            snaps.append(dict(ctx.get_state(), time=rec.time, dt=rec.dt, p=plug.est.copy(),
+                              res=plug.coupling_residual().copy(), vol=rec.volumes.copy(),
+                              art_flow=plug.models[0].results["ArtFlow"], repeats=rec.repeats))
+            print(f"t={rec.time:.4f} dt={rec.dt:.4f} repeats={rec.repeats} p={plug.est[0]:.5f} mmHg "
+                  f"V={rec.volumes[0] * 1e6:.6f} ml |V_FE - V_circ|={plug.coupling_residual()[0]:.1e} ml "
+                  f"newton={st.newton_iterations}", flush=True)
+
+        recs = solver.run(ctx, "newmark", solver.Timing(start=0.0, stop=0.06, dt=dt), plugins=[plug],
+                          on_step=on_step)
+    assert len(recs) >= 13
+    # ejection happened inside the window and the coupling met its tolerance after preloading
+    assert max(s["art_flow"] for s in snaps[3:]) > 0.1
+    for s in snaps[3:]:
+        assert s["res"][0] <= 1e-7
+    # mechanics parity of every accepted step, from the GPU's own state before it
+    cpu = CpuMechanics(mesh, mats)
+    uP = circulation.PRESSURE_UNITS["mmHg"]
+    worst = 0.0
+    # the plugin zeroes velocity and acceleration in the step after the last phase-1
+    # preloading step (CBCirculation.cpp:381-385), before that step's solve
+    zeroed = {round(t / dt) + 1 for (t, _, kind, _) in plug.log if kind == "preloading 1, last step"}
+    for k in range(1, len(snaps)):
+        pre, post = snaps[k - 1], snaps[k]
+        cpu.nodes = pre["nodes"].copy()
+        cpu.vel = pre["velocity"].copy()
+        cpu.acc = pre["acceleration"].copy()
+        if round(post["time"] / dt) in zeroed and k == 2:
+            cpu.vel[:] = 0.0
+            cpu.acc[:] = 0.0
+        cpu.pressures[:] = uP * post["p"][0]
+        # the oracle's Newton starts from the GPU's increment: the GPU's root must be a
+        # root of the oracle's equations (the oracle has no domain restarts of its own)
+        inc = post["nodes"] - pre["nodes"]
+        cpu._pending = dict(d=inc.copy(), dt=post["dt"])
+        it, rn = cpu.newmark_solve(post["time"], post["dt"], from_trial=True)
+        cpu.commit_step()
+        assert it <= 3, (k, it)
+        err = np.linalg.norm(cpu.nodes - post["nodes"]) / max(np.linalg.norm(inc), 1e-30)
+        vo = cpu.cavity_volumes()[0]
+        worst = max(worst, err)
+        assert err <= 1e-7, (k, err)
+        assert abs(vo - post["vol"][0]) <= 1e-9 * abs(vo), k
+    print(f"worst per-step increment error vs oracle: {worst:.2e}")

@@ -9,8 +9,10 @@ import subprocess
 import numpy as np
 import pytest
 
+from cardiomechanics_amd import circulation as CI
 from cardiomechanics_amd import materials as MT
 from cardiomechanics_amd import mesh as M
+from cardiomechanics_amd import settings as ST
 from cardiomechanics_amd import solver as SV
 from cardiomechanics_amd.context import MechanicsContext
 
@@ -38,3 +40,42 @@ def test_cmg_driver_cube_static_matches_python_host(tmp_path):
     ref = [r.volumes[0] for r in recs]
     np.testing.assert_allclose(vols, ref, rtol=1e-9)
     assert os.path.exists(tmp_path / "cube_pressure.dat")
+
+
+def test_cmg_driver_lv_windkessel_matches_python_host(tmp_path, monkeypatch):
+    """LV coupled to a three-element Windkessel (Plugins.Circulation) through the host C++
+    layer and through the Python host layer on the same files: the same cavity pressures
+    and volumes at every accepted step."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("cmg_driver not built")
+    mesh = M.lv_mesh(dims=(2, 4, 8), endo_normal="wall")
+    M.write_tetgen(str(tmp_path / "lv"), mesh)
+    M.write_lat(str(tmp_path / "lv.lat"), mesh.activation)
+    xml = os.path.join(ROOT, "cardiomechanics_amd", "configs", "lv_windkessel.xml")
+    shutil.copy(xml, tmp_path / "lv.xml")
+    out = subprocess.run([DRIVER, "lv.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(out.stdout[-3000:], out.stderr[-2000:])
+    assert out.returncode == 0
+    rows = [ln for ln in out.stdout.splitlines() if ln.startswith("t=")]
+    c_vol = np.array([float(re.search(r"volume=([-0-9.e+]+)", ln).group(1)) for ln in rows])
+    c_p = np.array([float(re.search(r"pressure=([-0-9.e+]+)", ln).group(1)) for ln in rows])
+    c_rep = [int(m.group(1)) if (m := re.search(r"repeats=(\d+)", ln)) else 0 for ln in rows]
+    assert os.path.exists(tmp_path / "lv_windkessel.dat")
+    # the Python host layer on the same files and settings
+    p = ST.parse_xml(str(tmp_path / "lv.xml"))
+    m2 = M.read_tetgen(str(tmp_path / "lv"))
+    m2.activation = M.read_lat(str(tmp_path / "lv.lat"), m2.n_elements)
+    prec = float(p["Solver.Precision"])
+    monkeypatch.chdir(tmp_path)   # the circulation writes its result files into the working directory
+    with MechanicsContext(m2, [MT.from_parameters(p, 0)], newton_atol=prec, newton_rtol=prec,
+                          newton_stol=prec) as ctx:
+        plug = CI.Circulation(p, groups={int(s): k for k, s in enumerate(ctx.group_ids)})
+        pressures = []
+        recs = SV.run(ctx, "newmark", SV.Timing(float(p["Solver.StartTime"]), float(p["Solver.StopTime"]),
+                                                float(p["Solver.TimeStep"])),
+                      plugins=[plug], on_step=lambda rec, st: pressures.append(plug.uP * plug.est[0]))
+    assert len(recs) == len(rows)
+    assert [r.repeats for r in recs] == c_rep
+    np.testing.assert_allclose(c_p, pressures, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(c_vol, [r.volumes[0] for r in recs], rtol=1e-9)
+    assert max(c_p) > 200.0     # the cavity pressure rose past the arterial pressure (2 mmHg)

@@ -59,6 +59,8 @@ CASES = {
     "ellipsoid_fixture_holzapfel": lambda: (ellipsoid_fixture(), [MT.lv_holzapfel()]),
     "ellipsoid_fixture_guccione": lambda: (ellipsoid_fixture(), [MT.cube_guccione()]),
     "cube_t4_guccione": lambda: (M.cube_mesh(2, order=4), [MT.cube_guccione()]),
+    "lv_holzapfel_tanh": lambda: (M.lv_mesh(dims=(2, 4, 8)), [MT.with_tanh(MT.lv_holzapfel(), PeakTension=60.0)]),
+    "cube_t4_guccione_tanh": lambda: (M.cube_mesh(2, order=4), [MT.with_tanh(MT.cube_guccione())]),
 }
 
 
@@ -91,12 +93,16 @@ def test_element_tangents_match_complex_step(case):
     assert rel.max() <= 1e-12
 
 
+@pytest.mark.parametrize("tension", ["sinus", "tanh"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("variant", [1, 3, 4])
-def test_element_kernel_variants_agree_with_the_oracle(variant, precision):
+def test_element_kernel_variants_agree_with_the_oracle(variant, precision, tension):
     """every residual+tangent kernel variant against the complex-step oracle (and the
-    residual against the oracle residual), T10 LV with Holzapfel and active tension"""
+    residual against the oracle residual), T10 LV with Holzapfel and active tension
+    (FromFunction sin^2 or the stretch-dependent TanH model)"""
     mesh, mats = lv_small()
+    if tension == "tanh":
+        mats = [MT.with_tanh(mats[0], PeakTension=60.0)]
     x = deformed(mesh)
     with MechanicsContext(mesh, mats, element_variant=variant, precision=precision) as ctx:
         Kg = ctx.element_tangents(x, 0.12).astype(np.float64)

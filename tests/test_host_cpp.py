@@ -15,7 +15,8 @@ def test_host_layer_selftest(tmp_path):
     exe = tmp_path / "host_selftest"
     subprocess.check_call(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I",
                            os.path.join(PKG, "host"), os.path.join(ROOT, "tests", "host", "host_selftest.cpp"),
-                           os.path.join(PKG, "host", "cm_host.cpp"), "-L", PKG, "-lcmg",
+                           os.path.join(PKG, "host", "cm_host.cpp"), os.path.join(PKG, "host", "cm_circulation.cpp"),
+                           "-L", PKG, "-lcmg",
                            f"-Wl,-rpath,{PKG}", "-o", str(exe)])
     out = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True)
     print(out.stdout)

@@ -132,6 +132,59 @@ def test_active_stress_is_derivative_of_T_sqrt_I4f():
     np.testing.assert_allclose(O.active_stress(np.eye(3), T), np.diag([T, 0, 0]))
 
 
+def _tanh_mat(**kw):
+    return MT.with_tanh(MT.cube_guccione(), **kw)
+
+
+def test_tanh_tension_limits_and_window():
+    """TanH tension (CBTensionModelTanH.cpp:71-97): zero before activation + EMD and at
+    the end of the duration, zero for fibre stretch at or below l0, periodic in the
+    cycle length, and in the saturated limit (t_s >> tau_c, t_dur - t_s >> tau_r) equal
+    to Tmax S_peak tanh(ld (lambda - l0))."""
+    m = _tanh_mat()
+    emd, dur = 0.015, 0.3
+    assert O.tanh_tension(m, 1.1, 0.010, 0.0) == 0.0              # t - t_act < EMD
+    assert O.tanh_tension(m, 1.1, 0.2 + emd - 1e-9, 0.2) == 0.0    # just before onset of an element at 0.2 s
+    assert O.tanh_tension(m, 1.1, emd + dur, 0.0) == 0.0           # t_s = t_dur
+    assert O.tanh_tension(m, 1.1, 0.2, 0.0) > 0.0
+    assert O.tanh_tension(m, 0.7, 0.2, 0.0) == 0.0                 # phi = 0
+    assert O.tanh_tension(m, 0.65, 0.2, 0.0) == 0.0                # phi < 0 clipped
+    for t in (0.05, 0.13, 0.27):
+        assert O.tanh_tension(m, 1.05, t + 1.0, 0.0) == pytest.approx(O.tanh_tension(m, 1.05, t, 0.0), rel=1e-12)
+    assert O.tanh_tension(m, 1.05, 0.13, 0.0) == O.tanh_tension(_tanh_mat(start_time=0.02), 1.05, 0.15, 0.0)
+    sat = _tanh_mat(Duration=20.0, ContractionTime=0.01, RelaxationTime=0.01, LengthDependenceUpstrokeTime=0.0)
+    for lam in (0.8, 1.0, 1.2):
+        got = O.tanh_tension(sat, lam, 10.0, 0.0)
+        assert got == pytest.approx(1000.0 * 120.0 * math.tanh(5.0 * (lam - 0.7)), rel=1e-14)
+    # monotone in stretch (length dependence) inside the window
+    lams = np.linspace(0.75, 1.3, 12)
+    vals = [O.tanh_tension(m, l, 0.1, 0.0) for l in lams]
+    assert np.all(np.diff(vals) > 0)
+
+
+def test_tanh_active_stress_and_element_tangent():
+    """The TanH active PK2 is kff T(lambda)/lambda on ff only; the element's complex-step
+    tangent through the stretch-dependent tension matches the reference's central-
+    difference tangent (transposed, CBElementSolidT10.cpp:511-567)."""
+    m = _tanh_mat()
+    rng = np.random.default_rng(5)
+    F = np.eye(3) + 0.1 * rng.standard_normal((3, 3))
+    S0 = O.active_stress_model(m, F, 0.1)
+    lam = np.linalg.norm(F[:, 0])
+    assert S0[0, 0] == pytest.approx(O.tanh_tension(m, lam, 0.1, 0.0) / lam, rel=1e-14)
+    assert np.count_nonzero(S0) == 1
+    X, x, dN, bases, w = element_case(np.random.default_rng(11))   # own stream: the module RNG feeds later tests
+    bc = np.zeros(30, np.uint8)
+    rc, Kcs = O.t10_cs_tangent(x.ravel(), dN, bases, w, bc, m, time=0.1)
+    rc2, Kfd = O.t10_fd_tangent(x.ravel(), dN, bases, w, bc, m, time=0.1, eps=1e-8)
+    assert rc == rc2 == O.SUCCESS
+    np.testing.assert_allclose(Kfd.T, Kcs, rtol=0, atol=1e-5 * np.abs(Kcs).max())
+    # the active tension contributes to the tangent (phi ~ 0 switches it off)
+    flat = _tanh_mat(DegreeOfLengthDependence=1e-9)
+    rc, Kflat = O.t10_cs_tangent(x.ravel(), dN, bases, w, bc, flat, time=0.1)
+    assert np.abs(Kflat - Kcs).max() > 1e-3 * np.abs(Kcs).max()
+
+
 def test_pk2_tangent_complex_step_matches_central_difference():
     for mat in (GUC, HO):
         F = rand_F(RNG, 0.1)
