@@ -98,6 +98,7 @@ class MechanicsContext:
         self._check(self.L.cmg_sizes(self.h, ctypes.byref(n_nodes), ctypes.byref(n_el), ctypes.byref(n_dofs),
                                      ctypes.byref(nnzb)), "cmg_sizes")
         self.n_nodes, self.n_dofs, self.nnzb = n_nodes.value, n_dofs.value, nnzb.value
+        self.n_elements = n_el.value
         self.rowptr = np.zeros(self.n_nodes + 1, np.int64)
         self.colind = np.zeros(self.nnzb, np.int32)
         self._check(self.L.cmg_pattern(self.h, _ptr(self.rowptr), _ptr(self.colind)), "cmg_pattern")
@@ -189,6 +190,14 @@ class MechanicsContext:
         xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64).ravel()
         self._check(self.L.cmg_cavity_volumes(self.h, _ptr(xa), _ptr(v)), "cmg_cavity_volumes")
         return v
+
+    def element_export(self, x=None):
+        """(F at the centroid, global frame (E,3,3); passive PK2 averaged over the
+        quadrature points, fibre frame (E,3,3)) at x or the current nodes"""
+        out = np.zeros((18, self.n_elements))
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64).ravel()
+        self._check(self.L.cmg_element_export(self.h, _ptr(xa), _ptr(out)), "cmg_element_export")
+        return out[:9].T.reshape(-1, 3, 3), out[9:].T.reshape(-1, 3, 3)
 
     def mass_matrix(self):
         v = np.zeros(self.nnzb * 9)

@@ -1092,6 +1092,23 @@ int cmg_cavity_volumes(cmg_ctx* c, const double* x, double* volumes) {
     return group_volumes(c, xd, volumes);
 }
 
+int cmg_element_export(cmg_ctx* c, const double* x, double* out) {
+    if (!c || !out) return fail("cmg_element_export: null argument");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    const double* xd = c->nodes.as<double>();
+    if (x) {
+        CMG_HIP_CHECK(hipMemcpyAsync(c->xeval.p, x, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        xd = c->xeval.as<double>();
+    }
+    DevBuf buf;
+    CMG_TRY(alloc(buf, (size_t)18 * c->E * sizeof(double)));
+    launch_element_export(c->npe, elem_args(c, xd, 0.0), c->Xref.as<double>(), buf.as<double>(), c->stream);
+    CMG_HIP_CHECK(hipGetLastError());
+    CMG_HIP_CHECK(hipMemcpyAsync(out, buf.p, (size_t)18 * c->E * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
 int cmg_mass_matrix(cmg_ctx* c, double* values) {
     if (!c || !values) return fail("cmg_mass_matrix: null argument");
     // rows of constrained components are zero (CBElementSolidT10.cpp:386-389)

@@ -70,6 +70,8 @@ struct SurfArgs {
 };
 
 // ---- launchers (element_kernels.hip) ----
+// per-element export state: out[18][E] = F at the centroid (global frame), averaged passive PK2
+void launch_element_export(int npe, const ElemArgs& a, const double* Xref, double* out, hipStream_t s);
 void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s);
 // variant: 1 = per-lane quadrature points, 2 = shared stage (LDS), 3 = shared stage, two node passes,
 // 4 = shared stage with two lanes (two waves) per element and spatial pair

@@ -986,7 +986,79 @@ __global__ void k_t10_shape(int64_t E, const int32_t* conn, const double* Xg, do
     w[e] = detJ / 24.0;
 }
 
+// Per-element state for export, one thread per element: the passive PK2 stress in
+// the fibre frame averaged over the quadrature points (CBElementSolidT10::GetPK2Stress,
+// CBElementSolidT10.cpp:817-838, (e1 + e2 + e3 + e4) / 4; T4: its one point) and the
+// deformation gradient at the centroid in the global frame (T10: dN/dX at the
+// centroid from the reference coordinates, CBElementSolidT10.cpp:307-322 before its
+// rotation; T4: the element's constant F).  out[r][E]: r < 9 F, 9 <= r < 18 S (NaN for
+// an element with det F <= 0).
+template <int NPE, int NQP>
+__global__ void __launch_bounds__(128) k_element_export(ElemArgs a, const double* Xref, double* out) {
+#pragma clang fp contract(off)
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const int64_t E = a.E;
+    const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
+    double x[3 * NPE];
+    for (int k = 0; k < NPE; k++) {
+        const int64_t n = a.conn[(int64_t)k * E + e];
+        x[3 * k] = a.x[3 * n];
+        x[3 * k + 1] = a.x[3 * n + 1];
+        x[3 * k + 2] = a.x[3 * n + 2];
+    }
+    double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool ok = true;
+    for (int q = 0; q < NQP; q++) {
+        double g[NPE][3];
+        load_g<NPE>(a, e, q, g);
+        double B[9], Binv[9], Floc[9];
+        for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
+        local_F<NPE>(x, g, B, Binv, Floc);
+        Law L;
+        if (!eval_law(m, Floc, 0.0, 0.0, false, L)) ok = false;
+        for (int r = 0; r < 9; r++) S[r] += L.S[r];
+    }
+    double F[9];
+    if (NPE == 10) {
+        double X[30], dN[30];
+        for (int k = 0; k < 10; k++) {
+            const int64_t n = a.conn[(int64_t)k * E + e];
+            for (int c = 0; c < 3; c++) X[3 * k + c] = Xref[3 * n + c];
+        }
+        t10_shape_point(X, 0.25, 0.25, 0.25, 0.25, dN);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double t = 0.0;
+                for (int k = 0; k < 10; k++) t += dN[3 * k + j] * x[3 * k + i];
+                F[3 * i + j] = t;
+            }
+    } else {
+        double g[NPE][3];
+        load_g<NPE>(a, e, 0, g);
+        for (int i = 0; i < 3; i++) {
+            // CBElementSolidT4::CalcDeformationTensorWithLocalBasis (:130-144), before the rotation
+            F[3 * i] = g[0][0] * x[i] + g[1][0] * x[3 + i] + g[2][0] * x[6 + i] + g[3][0] * x[9 + i];
+            F[3 * i + 1] = g[0][1] * x[i] + g[1][1] * x[3 + i] + g[2][1] * x[6 + i] + g[3][1] * x[9 + i];
+            F[3 * i + 2] = g[0][2] * x[i] + g[1][2] * x[3 + i] + g[2][2] * x[6 + i] + g[3][2] * x[9 + i];
+        }
+    }
+    for (int r = 0; r < 9; r++) {
+        out[(int64_t)r * E + e] = F[r];
+        out[(int64_t)(9 + r) * E + e] = ok ? S[r] / NQP : NAN;
+    }
+}
+
 }  // namespace
+
+void launch_element_export(int npe, const ElemArgs& a, const double* Xref, double* out, hipStream_t s) {
+    if (a.E == 0) return;
+    const int nblk = (int)((a.E + 127) / 128);
+    if (npe == 10)
+        hipLaunchKernelGGL((k_element_export<10, 4>), dim3(nblk), dim3(128), 0, s, a, Xref, out);
+    else
+        hipLaunchKernelGGL((k_element_export<4, 1>), dim3(nblk), dim3(128), 0, s, a, Xref, out);
+}
 
 void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s) {
     if (a.E == 0) return;

@@ -204,6 +204,16 @@ int cmg_cavity_volumes(cmg_ctx* ctx, const double* x, double* volumes);
 /* consistent (or lumped) mass matrix in the block pattern
  * (CBElementSolidT10.cpp:352-422, CBSolverNewmarkBeta.cpp:704-787) */
 int cmg_mass_matrix(cmg_ctx* ctx, double* values);
+/* Per-element state for result export (CBSolver::Export, CBSolver.cpp:538-585), at x
+ * (host [3N]) or, with x = NULL, at the current nodes.  out[18][E] (row r, element e
+ * at out[r*E + e]): rows 0-8 the deformation gradient at the element centroid in the
+ * global frame (CBElementSolidT10::GetDeformationTensor, CBElementSolidT10.cpp:764-777,
+ * before its rotation into the fibre frame; T4: CBElementSolidT4.cpp:232-245), rows
+ * 9-17 the passive PK2 stress in the fibre frame averaged over the quadrature points
+ * (CBElementSolidT10::GetPK2Stress, :817-838), NaN for an element with det F <= 0.
+ * J, fibre/sheet/normal stretch (Lambda), C, E and the rotated fibre follow from F
+ * and the centroid basis on the host. */
+int cmg_element_export(cmg_ctx* ctx, const double* x, double* out);
 
 /* device-pointer variants (stream ordered) */
 int cmg_residual_device(cmg_ctx* ctx, const double* x_dev, double time, int32_t include_pressure, double* f_dev);
