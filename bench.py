@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--beat-steps", type=int, default=2, help="Newmark steps of the beat to time (0: skip)")
     ap.add_argument("--krylov", default="bicgstab,bicgstab_graph",
                     help="Krylov methods to time the beat steps with (comma list of bicgstab, bicgstab_graph, gmres)")
+    ap.add_argument("--beat-mixed", type=int, default=1,
+                    help="also time the beat steps in mixed precision (BiCGStab) and report their displacement "
+                         "difference to the FP64 run (0: off)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="CPU baseline sample: repeat 20k-element chunks of the mesh for about this long")
     ap.add_argument("--no-cpu", action="store_true")
@@ -167,13 +170,20 @@ def main():
     beat = None
     if args.beat_steps > 0:
         beat = {}
-        for name in args.krylov.split(","):
-            method = {"gmres": 0, "bicgstab_graph": 1, "bicgstab": 2}[name.strip()]
-            bctx = MechanicsContext(mesh, mats, precision=prec, element_variant=best, krylov_method=method)
+        runs = [(name.strip(), 0) for name in args.krylov.split(",")]
+        if args.beat_mixed:
+            runs.append(("bicgstab", 1))
+        final = {}
+        for name, bprec in runs:
+            method = {"gmres": 0, "bicgstab_graph": 1, "bicgstab": 2}[name]
+            # the element variant measured above for FP64; the precision's default otherwise
+            bctx = MechanicsContext(mesh, mats, precision=bprec, krylov_method=method,
+                                    **({"element_variant": best} if bprec == prec else {}))
             plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
             dt = 1e-3
             times = []
             its = []
+            key = name if bprec == 0 else name + "_mixed"
             for k in range(1, args.beat_steps + 1):
                 tk = k * dt
                 plug.apply(tk, dt)
@@ -182,11 +192,23 @@ def main():
                 rc, st = bctx.newmark_step(tk, dt)
                 times.append(time.perf_counter() - b0)
                 its.append((st.newton_iterations, st.krylov_iterations))
-                log(f"[bench] {name} newmark step t={tk * 1e3:.0f} ms rc={rc} newton={st.newton_iterations} "
+                log(f"[bench] {key} newmark step t={tk * 1e3:.0f} ms rc={rc} newton={st.newton_iterations} "
                     f"krylov={st.krylov_iterations} {times[-1]:.3f} s")
+            ms, lay, lanes = bctx.spmv_tuning()
+            log(f"[bench] {key} SpMV layouts (ms): block {ms[0]:.3f}/{ms[1]:.3f}/{ms[2]:.3f}, "
+                f"chunk {ms[3]:.3f}/{ms[4]:.3f}/{ms[5]:.3f}, flat {ms[6]:.3f}/{ms[7]:.3f}/{ms[8]:.3f}; "
+                f"chosen layout {lay} x {lanes} lanes")
+            final[key] = bctx.get_state()["nodes"]
             bctx.close()
-            beat[name] = {"steps": args.beat_steps, "dt": dt, "s_per_step": float(np.mean(times)),
-                          "newton_krylov": its, "beat_800_steps_s_extrapolated": float(np.mean(times) * 800)}
+            beat[key] = {"steps": args.beat_steps, "dt": dt, "precision": "mixed" if bprec else "fp64",
+                         "s_per_step": float(np.mean(times)),
+                         "newton_krylov": its, "beat_800_steps_s_extrapolated": float(np.mean(times) * 800),
+                         "spmv_ms": [float(v) for v in ms], "spmv_layout": [lay, lanes]}
+        if "bicgstab_mixed" in final and "bicgstab" in final:
+            u = final["bicgstab"] - mesh.coords.ravel()
+            d = float(np.linalg.norm(final["bicgstab_mixed"] - final["bicgstab"]) / np.linalg.norm(u))
+            beat["bicgstab_mixed"]["rel_displacement_difference_vs_fp64"] = d
+            log(f"[bench] mixed vs FP64 displacement after {args.beat_steps} steps: {d:.2e} (relative)")
 
     cpu = None
     if not args.no_cpu:

@@ -22,7 +22,7 @@ EXPORTED = [
     "cmg_default_options", "cmg_last_error", "cmg_device_count", "cmg_create", "cmg_destroy", "cmg_sizes",
     "cmg_pattern", "cmg_set_pressures", "cmg_synchronize", "cmg_stream", "cmg_residual", "cmg_tangent",
     "cmg_element_residuals", "cmg_element_tangents", "cmg_surface_forces", "cmg_cavity_volumes",
-    "cmg_mass_matrix", "cmg_element_export", "cmg_residual_device", "cmg_residual_tangent_device", "cmg_element_kernel_device",
+    "cmg_mass_matrix", "cmg_element_export", "cmg_spmv_tuning", "cmg_residual_device", "cmg_residual_tangent_device", "cmg_element_kernel_device",
     "cmg_spmv", "cmg_spmv_layout", "cmg_linear_solve", "cmg_set_state", "cmg_get_state", "cmg_set_initial_guess",
     "cmg_static_step", "cmg_newmark_step", "cmg_newmark_solve", "cmg_trial_volumes", "cmg_commit_step",
     "cmg_copy_device",
@@ -124,6 +124,7 @@ def load():
         "cmg_cavity_volumes": (ctypes.c_int, [P, P, P]),
         "cmg_mass_matrix": (ctypes.c_int, [P, P]),
         "cmg_element_export": (ctypes.c_int, [P, P, P]),
+        "cmg_spmv_tuning": (ctypes.c_int, [P, P, P, P]),
         "cmg_residual_device": (ctypes.c_int, [P, P, dbl, i32, P]),
         "cmg_residual_tangent_device": (ctypes.c_int, [P, P, dbl, i32, P, P]),
         "cmg_element_kernel_device": (ctypes.c_int, [P, P, dbl, i32]),

@@ -204,16 +204,25 @@ class MechanicsContext:
         self._check(self.L.cmg_mass_matrix(self.h, _ptr(v)), "cmg_mass_matrix")
         return v
 
-    def spmv(self, values, x, lanes=None, chunked=False):
+    def spmv(self, values, x, lanes=None, layout=0):
+        """y = A x; lanes/layout select the kernel (layout 0 block, 1 chunked, 2 flat)"""
         values = np.ascontiguousarray(values, dtype=np.float64)
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(self.n_dofs)
         if lanes is None:
             self._check(self.L.cmg_spmv(self.h, _ptr(values), _ptr(x), _ptr(y)), "cmg_spmv")
         else:
-            self._check(self.L.cmg_spmv_layout(self.h, _ptr(values), _ptr(x), _ptr(y), lanes, int(chunked)),
+            self._check(self.L.cmg_spmv_layout(self.h, _ptr(values), _ptr(x), _ptr(y), lanes, int(layout)),
                         "cmg_spmv_layout")
         return y
+
+    def spmv_tuning(self):
+        """(ms per product of the 9 layouts, chosen layout, lanes) after the first solve"""
+        ms = np.zeros(9)
+        lay, lanes = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.L.cmg_spmv_tuning(self.h, _ptr(ms), ctypes.byref(lay), ctypes.byref(lanes)),
+                    "cmg_spmv_tuning")
+        return ms, lay.value, lanes.value
 
     def linear_solve(self, values, b, x0=None, rtol=1e-10):
         values = np.ascontiguousarray(values, dtype=self.value_dtype)

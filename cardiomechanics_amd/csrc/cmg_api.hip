@@ -585,7 +585,7 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
     CMG_TRY(norm2(c, b, &bnorm));
     // r = b - A x
     launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s, c->spmv_lanes,
-                c->spmv_chunked);
+                c->spmv_layout);
     launch_waxpby(n, 1.0, b, -1.0, w, V, s);
     double beta;
     CMG_TRY(norm2(c, V, &beta));
@@ -611,7 +611,7 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
             double* vn = V + (int64_t)(j + 1) * n;
             launch_block_jacobi_apply(c->N, c->dinv.as<double>(), vj, z, s);
             launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, z, vn, s, c->spmv_lanes,
-                        c->spmv_chunked);
+                        c->spmv_layout);
             // pass 1: h_i = v_i . w (i <= j) and ||w||^2 in one multi-dot
             CMG_TRY(multidot(c, j + 2, V, n, vn, h.data()));
             const double wn2 = h[j + 1];
@@ -667,7 +667,7 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
         launch_axpby(n, 1.0, z, 1.0, x, s);
         // true residual
         launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s, c->spmv_lanes,
-                c->spmv_chunked);
+                c->spmv_layout);
         launch_waxpby(n, 1.0, b, -1.0, w, V, s);
         CMG_TRY(norm2(c, V, &beta));
         st->iterations = total;
@@ -707,7 +707,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
     launch_block_jacobi_setup(c->N, rp, ci, vals, c->fp32, c->dinv.as<double>(), s);
     double bnorm;
     CMG_TRY(norm2(c, b, &bnorm));
-    launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s, c->spmv_lanes, c->spmv_chunked);
+    launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s, c->spmv_lanes, c->spmv_layout);
     launch_waxpby(n, 1.0, b, -1.0, v, r, s);
     double rn;
     CMG_TRY(norm2(c, r, &rn));
@@ -736,7 +736,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
         }
         rho = rho_new;
         launch_block_jacobi_apply(c->N, c->dinv.as<double>(), p, ph, s);
-        launch_spmv(c->N, rp, ci, vals, c->fp32, ph, v, s, c->spmv_lanes, c->spmv_chunked);
+        launch_spmv(c->N, rp, ci, vals, c->fp32, ph, v, s, c->spmv_lanes, c->spmv_layout);
         double d0;
         CMG_TRY(multidot(c, 1, rh, 0, v, &d0));
         bool breakdown = !(std::fabs(d0) > 0.0) || !std::isfinite(d0);
@@ -744,7 +744,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
             alpha = rho / d0;
             launch_waxpby(n, 1.0, r, -alpha, v, sv, s);   // s = r - alpha v
             launch_block_jacobi_apply(c->N, c->dinv.as<double>(), sv, sh, s);
-            launch_spmv(c->N, rp, ci, vals, c->fp32, sh, t, s, c->spmv_lanes, c->spmv_chunked);
+            launch_spmv(c->N, rp, ci, vals, c->fp32, sh, t, s, c->spmv_lanes, c->spmv_layout);
             launch_multidot(n, 3, rh, n, t, part, s);      // (rh,t) (s,t) (t,t)
             launch_reduce_partials(nb, 3, part, dh, s);
             launch_multidot(n, 2, rh, n, sv, part, s);     // (rh,s) (s,s)
@@ -764,7 +764,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
         }
         if (rn <= target || breakdown) {
             // the recurrence residual drifts from b - A x: confirm, restart if needed
-            launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s, c->spmv_lanes, c->spmv_chunked);
+            launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s, c->spmv_lanes, c->spmv_layout);
             launch_waxpby(n, 1.0, b, -1.0, v, r, s);
             CMG_TRY(norm2(c, r, &rn));
             if (rn <= target || ++restarts > 50 || !std::isfinite(rn)) break;
@@ -786,24 +786,30 @@ int tune_spmv(cmg_ctx* c, const void* vals, const double* b) {
     hipEvent_t e0, e1;
     CMG_HIP_CHECK(hipEventCreate(&e0));
     CMG_HIP_CHECK(hipEventCreate(&e1));
-    const int lanes[3] = {4, 8, 16};
-    float best = 1e30f;
+    const int lanes[3][3] = {{4, 8, 16}, {4, 8, 16}, {16, 32, 64}};
+    float best = 1e30f, best_blk = 1e30f;
     const int64_t* rp = c->rowptr.as<int64_t>();
     const int32_t* ci = c->colind.as<int32_t>();
-    for (int chunked = 0; chunked < 2; chunked++)
-        for (int l : lanes) {
-            launch_spmv(c->N, rp, ci, vals, c->fp32, b, c->tmpA.as<double>(), s, l, chunked);
+    for (int layout = kSpmvBlock; layout <= kSpmvFlat; layout++)
+        for (int li = 0; li < 3; li++) {
+            const int l = lanes[layout][li];
+            launch_spmv(c->N, rp, ci, vals, c->fp32, b, c->tmpA.as<double>(), s, l, layout);
             CMG_HIP_CHECK(hipEventRecord(e0, s));
-            for (int k = 0; k < 2; k++) launch_spmv(c->N, rp, ci, vals, c->fp32, b, c->tmpA.as<double>(), s, l, chunked);
+            for (int k = 0; k < 2; k++) launch_spmv(c->N, rp, ci, vals, c->fp32, b, c->tmpA.as<double>(), s, l, layout);
             CMG_HIP_CHECK(hipEventRecord(e1, s));
             CMG_HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
             CMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            c->spmv_ms[chunked * 3 + (l == 4 ? 0 : l == 8 ? 1 : 2)] = 0.5 * ms;
+            c->spmv_ms[layout * 3 + li] = 0.5 * ms;
             if (ms < best) {
                 best = ms;
                 c->spmv_lanes = l;
-                c->spmv_chunked = chunked != 0;
+                c->spmv_layout = layout;
+            }
+            if (layout != kSpmvFlat && ms < best_blk) {
+                best_blk = ms;
+                c->bicg_lanes = l;
+                c->bicg_chunked = layout == kSpmvChunk;
             }
         }
     (void)hipEventDestroy(e0);
@@ -861,13 +867,13 @@ int bicgstab_device(cmg_ctx* c, const void* vals, const double* b, double* x, do
     launch_fill_int(1, 1, active, s);
     auto restart = [&]() {
         // r = b - A x, then r^ = r, p = r and the scalars
-        launch_spmv(c->N, rp, ci, vals, c->fp32, x, k.r, s, c->spmv_lanes, c->spmv_chunked);
+        launch_spmv(c->N, rp, ci, vals, c->fp32, x, k.r, s, c->spmv_lanes, c->spmv_layout);
         launch_waxpby(n, 1.0, b, -1.0, k.r, k.r, s);
         launch_bicg_init(k, c->N, b, rtol_dev, active, s);
     };
     restart();
     auto iterate = [&]() {
-        launch_bicg_iteration(k, c->spmv_lanes, c->spmv_chunked, c->N, rp, ci, vals, c->fp32, 0,
+        launch_bicg_iteration(k, c->bicg_lanes, c->bicg_chunked, c->N, rp, ci, vals, c->fp32, 0,
                               c->dinv.as<double>(), s);
     };
     const bool graph_ok = c->bicg_graph != nullptr && c->bicg_graph_x == x && c->bicg_graph_vals == vals;
@@ -913,7 +919,7 @@ int bicgstab_device(cmg_ctx* c, const void* vals, const double* b, double* x, do
             continue;
         }
         // confirm with the true residual (the recurrence drifts from b - A x)
-        launch_spmv(c->N, rp, ci, vals, c->fp32, x, k.r, s, c->spmv_lanes, c->spmv_chunked);
+        launch_spmv(c->N, rp, ci, vals, c->fp32, x, k.r, s, c->spmv_lanes, c->spmv_layout);
         launch_waxpby(n, 1.0, b, -1.0, k.r, k.r, s);
         CMG_TRY(norm2(c, k.r, &rn));
         if (rn <= target) {
@@ -925,7 +931,7 @@ int bicgstab_device(cmg_ctx* c, const void* vals, const double* b, double* x, do
         launch_bicg_init(k, c->N, b, rtol_dev, active, s);   // restart from the current x
     }
     if (!st->converged) {   // final residual of an unconverged solve
-        launch_spmv(c->N, rp, ci, vals, c->fp32, x, k.r, s, c->spmv_lanes, c->spmv_chunked);
+        launch_spmv(c->N, rp, ci, vals, c->fp32, x, k.r, s, c->spmv_lanes, c->spmv_layout);
         launch_waxpby(n, 1.0, b, -1.0, k.r, k.r, s);
         CMG_TRY(norm2(c, k.r, &rn));
     }
@@ -1120,9 +1126,13 @@ int cmg_mass_matrix(cmg_ctx* c, double* values) {
     return CMG_SUCCESS;
 }
 
-int cmg_spmv_layout(cmg_ctx* c, const double* values, const double* x, double* y, int32_t lanes, int32_t chunked) {
+int cmg_spmv_layout(cmg_ctx* c, const double* values, const double* x, double* y, int32_t lanes, int32_t layout) {
     if (!c || !values || !x || !y) return fail("cmg_spmv: null argument");
-    if (lanes != 4 && lanes != 8 && lanes != 16) return fail("cmg_spmv_layout: lanes must be 4, 8 or 16");
+    if (layout < kSpmvBlock || layout > kSpmvFlat) return fail("cmg_spmv_layout: layout must be 0, 1 or 2");
+    if (layout != kSpmvFlat && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail("cmg_spmv_layout: lanes must be 4, 8 or 16 for layouts 0 and 1");
+    if (layout == kSpmvFlat && lanes != 16 && lanes != 32 && lanes != 64)
+        return fail("cmg_spmv_layout: lanes must be 16, 32 or 64 for layout 2");
     CMG_HIP_CHECK(hipSetDevice(c->device));
     const size_t vb = c->n * sizeof(double);
     DevBuf v64;   // padded for the chunked layout
@@ -1131,9 +1141,18 @@ int cmg_spmv_layout(cmg_ctx* c, const double* values, const double* x, double* y
     CMG_HIP_CHECK(hipMemcpyAsync(v64.p, values, (size_t)c->nnzb * 9 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CMG_HIP_CHECK(hipMemcpyAsync(c->tmpA.p, x, vb, hipMemcpyHostToDevice, c->stream));
     launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), v64.p, false, c->tmpA.as<double>(),
-                c->tmpB.as<double>(), c->stream, lanes, chunked != 0);
+                c->tmpB.as<double>(), c->stream, lanes, layout);
     CMG_HIP_CHECK(hipMemcpyAsync(y, c->tmpB.p, vb, hipMemcpyDeviceToHost, c->stream));
     CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+int cmg_spmv_tuning(cmg_ctx* c, double* ms, int32_t* layout, int32_t* lanes) {
+    if (!c) return fail("null context");
+    if (ms)
+        for (int k = 0; k < 9; k++) ms[k] = c->spmv_ms[k];
+    if (layout) *layout = c->spmv_lanes ? c->spmv_layout : 0;
+    if (lanes) *lanes = c->spmv_lanes;
     return CMG_SUCCESS;
 }
 

@@ -61,7 +61,11 @@ struct cmg_ctx {
     int n_groups = 0;
     int n_mats = 0;
     int spmv_lanes = 0;         // lanes per block row of the SpMV, 0 until timed on the first solve
-    bool spmv_chunked = false;  // 144-byte value chunks instead of one block per step
+    int spmv_layout = 0;        // kSpmvBlock / kSpmvChunk / kSpmvFlat
+    // the fastest block or chunk layout: the device-scalar BiCGStab fuses its SpMV with
+    // dot products and exists in those two layouts only
+    int bicg_lanes = 8;
+    bool bicg_chunked = false;
     // device-scalar BiCGStab: workspace and the graph of `bicg_chunk` iterations
     cmg::DevBuf bicg_part, bicg_sc, bicg_flags;
     hipGraphExec_t bicg_graph = nullptr;
@@ -69,7 +73,7 @@ struct cmg_ctx {
     const void* bicg_graph_vals = nullptr;
     int bicg_chunk = 16;
     bool bicg_use_graph = true;
-    double spmv_ms[6] = {0, 0, 0, 0, 0, 0};   // timed SpMV: one block at 4/8/16 lanes, chunks at 4/8/16
+    double spmv_ms[9] = {};     // timed SpMV: block at 4/8/16 lanes, chunks at 4/8/16, flat at 16/32/64
     bool fp32 = false;
     // host copies needed for maps / exports
     std::vector<uint8_t> h_bc;

@@ -100,11 +100,14 @@ void launch_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* cont
                           hipStream_t s, const SetStrides& b = SetStrides());   // strides: Ke, Ks, vals
 
 // ---- launchers (linalg_kernels.hip) ----
-// lanes per block row: 4, 8 or 16; chunked: 144-byte value chunks (needs 16-byte
-// aligned values and columns padded by kSpmvPadBlocks blocks)
+// SpMV layouts: kSpmvBlock one 3x3 block per lane step (4, 8 or 16 lanes per block
+// row); kSpmvChunk 144-byte value chunks (needs 16-byte aligned values and columns
+// padded by kSpmvPadBlocks blocks; 4, 8 or 16 lanes); kSpmvFlat the row's values as
+// one coalesced word stream (16, 32 or 64 lanes)
 constexpr int kSpmvPadBlocks = 4;
+constexpr int kSpmvBlock = 0, kSpmvChunk = 1, kSpmvFlat = 2;
 void launch_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals, bool fp32,
-                 const double* x, double* y, hipStream_t s, int lanes = 8, bool chunked = false);
+                 const double* x, double* y, hipStream_t s, int lanes = 8, int layout = kSpmvBlock);
 void launch_spmv_scalar_mass(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval,
                              const uint8_t* bcmask, const double* x, double* y, hipStream_t s);
 void launch_block_jacobi_setup(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals,

@@ -102,6 +102,52 @@ __global__ __launch_bounds__(256) void k_spmv_chunk(int64_t N, const int64_t* ro
     }
 }
 
+// y = A x with the values read as one flat stream: G lanes share a block row and
+// lane l takes the value words l, l + G, l + 2G, ... of the row's 9 nb values
+// ([block][i][j] order), so every load instruction of a wave reads consecutive
+// addresses (fully coalesced, unlike one 72-byte block per lane).  Each lane keeps
+// three partial sums (its words' rows i) and the group reduces them with shuffles.
+template <typename VT, int G>
+__global__ __launch_bounds__(256) void k_spmv_flat(int64_t N, const int64_t* rowptr, const int32_t* colind,
+                                                   const VT* vals, const double* x, double* y) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = gt / G;
+    const int sub = (int)(gt % G);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (r < N) {
+        const int64_t k0 = rowptr[r];
+        const int cnt = (int)(9 * (rowptr[r + 1] - k0));
+        const VT* v = vals + 9 * k0;
+        const int32_t* ci = colind + k0;
+        constexpr int GB = G / 9, GR = G % 9;
+        int blk = sub / 9, rem = sub - 9 * (sub / 9);
+        for (int idx = sub; idx < cnt; idx += G) {
+            const int i = rem / 3, j = rem - 3 * (rem / 3);
+            const double p = (double)v[idx] * x[3 * (int64_t)ci[blk] + j];
+            s0 += (i == 0) ? p : 0.0;
+            s1 += (i == 1) ? p : 0.0;
+            s2 += (i == 2) ? p : 0.0;
+            blk += GB;
+            rem += GR;
+            if (rem >= 9) {
+                rem -= 9;
+                blk++;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, G);
+        s1 += __shfl_xor(s1, o, G);
+        s2 += __shfl_xor(s2, o, G);
+    }
+    if (r < N && sub == 0) {
+        y[3 * r] = s0;
+        y[3 * r + 1] = s1;
+        y[3 * r + 2] = s2;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_spmv_scalar_mass(int64_t N, const int64_t* rowptr, const int32_t* colind,
                                                           const double* mval, const uint8_t* bcmask,
                                                           const double* x, double* y) {
@@ -292,11 +338,28 @@ inline dim3 grid1(int64_t n, int b = 256) { return dim3((unsigned)((n + b - 1) /
 }  // namespace
 
 void launch_spmv(int64_t N, const int64_t* rowptr, const int32_t* colind, const void* vals, bool fp32,
-                 const double* x, double* y, hipStream_t s, int lanes, bool chunked) {
+                 const double* x, double* y, hipStream_t s, int lanes, int layout) {
     if (N == 0) return;
+    if (layout == kSpmvFlat) {
+        if (lanes != 16 && lanes != 64) lanes = 32;
+        const dim3 g = grid1(N * lanes);
+#define CMG_SPMV(VT, L) \
+    hipLaunchKernelGGL((k_spmv_flat<VT, L>), g, dim3(256), 0, s, N, rowptr, colind, (const VT*)vals, x, y)
+        if (fp32) {
+            if (lanes == 16) CMG_SPMV(float, 16);
+            else if (lanes == 64) CMG_SPMV(float, 64);
+            else CMG_SPMV(float, 32);
+        } else {
+            if (lanes == 16) CMG_SPMV(double, 16);
+            else if (lanes == 64) CMG_SPMV(double, 64);
+            else CMG_SPMV(double, 32);
+        }
+#undef CMG_SPMV
+        return;
+    }
     if (lanes != 4 && lanes != 16) lanes = 8;
     const dim3 g = grid1(N * lanes);
-    if (chunked) {
+    if (layout == kSpmvChunk) {
 #define CMG_SPMV(VT, L) \
     hipLaunchKernelGGL((k_spmv_chunk<VT, L>), g, dim3(256), 0, s, N, rowptr, colind, (const VT*)vals, x, y)
         if (fp32) {

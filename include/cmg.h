@@ -229,11 +229,16 @@ int cmg_copy_device(cmg_ctx* ctx, const double* src_dev, double* dst_dev, int64_
 
 /* ---- linear algebra on the context's pattern ---------------------------- */
 int cmg_spmv(cmg_ctx* ctx, const double* values, const double* x, double* y);
-/* the same product with an explicit kernel layout: lanes per block row 4, 8 or 16;
- * chunked 1 reads the values in 144-byte chunks (the Krylov solvers pick the fastest
- * layout on their first solve) */
+/* the same product with an explicit kernel layout (the Krylov solvers pick the fastest
+ * on their first solve): layout 0 one 3x3 block per lane step, 1 the values read in
+ * 144-byte chunks (both with lanes = 4, 8 or 16 per block row), 2 the block row's
+ * values read as one coalesced word stream (lanes = 16, 32 or 64) */
 int cmg_spmv_layout(cmg_ctx* ctx, const double* values, const double* x, double* y, int32_t lanes,
-                    int32_t chunked);
+                    int32_t layout);
+/* the SpMV tuning of the context's first Krylov solve: ms[9] = time per product of
+ * layouts 0, 1 at 4/8/16 lanes and layout 2 at 16/32/64 lanes; the chosen layout and
+ * lanes (all zero before the first solve) */
+int cmg_spmv_tuning(cmg_ctx* ctx, double* ms, int32_t* layout, int32_t* lanes);
 /* BiCGStab or GMRES(m) (cmg_options.krylov_method), right preconditioned with 3x3
  * nodal block Jacobi; replaces the KSP/PC of CBSolverNewmarkBeta.cpp:166-196 (a
  * Krylov solve instead of LU). */

@@ -201,9 +201,9 @@ def test_spmv_and_krylov_solve(method):
         K = ctx.bsr(vals).tocsr()
         v = rng.normal(size=ctx.n_dofs)
         np.testing.assert_allclose(ctx.spmv(vals, v), K @ v, rtol=1e-13, atol=1e-13 * np.abs(K @ v).max())
-        for lanes in (4, 8, 16):
-            for chunked in (False, True):
-                np.testing.assert_allclose(ctx.spmv(vals, v, lanes, chunked), K @ v, rtol=1e-13,
+        for layout, lane_set in ((0, (4, 8, 16)), (1, (4, 8, 16)), (2, (16, 32, 64))):
+            for lanes in lane_set:
+                np.testing.assert_allclose(ctx.spmv(vals, v, lanes, layout), K @ v, rtol=1e-13,
                                            atol=1e-13 * np.abs(K @ v).max())
         b = K @ rng.normal(size=ctx.n_dofs)
         xs, st = ctx.linear_solve(vals, b, rtol=1e-12)
