@@ -164,6 +164,7 @@ void cmg_destroy(cmg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->bicg_graph) (void)hipGraphExecDestroy(c->bicg_graph);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->h_part) (void)hipHostFree(c->h_part);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -692,7 +693,7 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
     const int64_t n = c->n;
     const int nb = dot_blocks(n);
     double* W = c->gm_V.as<double>();   // (restart + 2) >= 10 vectors
-    double* rh = W;                     // rh, sv, t adjacent: one strided dot launch covers them
+    double* rh = W;
     double* sv = W + n;
     double* t = W + 2 * n;
     double* r = W + 3 * n;
@@ -701,9 +702,24 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
     double* ph = W + 6 * n;
     double* sh = W + 7 * n;
     double* part = c->gm_partial.as<double>();
-    double* dh = c->gm_h.as<double>();
+    const double* dinv = c->dinv.as<double>();
     const int64_t* rp = c->rowptr.as<int64_t>();
     const int32_t* ci = c->colind.as<int32_t>();
+    if (c->h_part_len < 5 * (int64_t)nb) {
+        if (c->h_part) CMG_HIP_CHECK(hipHostFree(c->h_part));
+        c->h_part = nullptr;
+        CMG_HIP_CHECK(hipHostMalloc((void**)&c->h_part, 5 * (size_t)nb * sizeof(double), hipHostMallocDefault));
+        c->h_part_len = 5 * (int64_t)nb;
+    }
+    // dot products: per-chunk partials copied to the host and summed there in chunk order
+    auto host_sums = [&](int k, double* out) -> int {
+        CMG_HIP_CHECK(hipMemcpyAsync(c->h_part, part, (size_t)k * nb * sizeof(double), hipMemcpyDeviceToHost, s));
+        CMG_HIP_CHECK(hipStreamSynchronize(s));
+        for (int j = 0; j < k; j++) out[j] = 0.0;
+        for (int blk = 0; blk < nb; blk++)
+            for (int j = 0; j < k; j++) out[j] += c->h_part[(int64_t)blk * k + j];
+        return CMG_SUCCESS;
+    };
     launch_block_jacobi_setup(c->N, rp, ci, vals, c->fp32, c->dinv.as<double>(), s);
     double bnorm;
     CMG_TRY(norm2(c, b, &bnorm));
@@ -724,39 +740,32 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
     bool fresh = true;
     double rho = 1.0, alpha = 1.0, omega = 1.0, rho_new = rn * rn;
     while (it < maxit) {
-        if (fresh) {   // (re)start: shadow residual = r, p = r
+        // p = r + beta (p - omega v) (p = r on a (re)start, with shadow residual r^ = r); ph = D^-1 p
+        double beta = 0.0;
+        if (fresh) {
             launch_scale_copy(n, 1.0, r, rh, s);
-            launch_scale_copy(n, 1.0, r, p, s);
             rho_new = rn * rn;
-            fresh = false;
-        } else {       // p = r + beta (p - omega v)
-            const double beta = (rho_new / rho) * (alpha / omega);
-            launch_axpby(n, -omega, v, 1.0, p, s);
-            launch_axpby(n, 1.0, r, beta, p, s);
+        } else {
+            beta = (rho_new / rho) * (alpha / omega);
         }
+        launch_bicg_p_bj(c->N, fresh, beta, omega, r, v, dinv, p, ph, s);
+        fresh = false;
         rho = rho_new;
-        launch_block_jacobi_apply(c->N, c->dinv.as<double>(), p, ph, s);
         launch_spmv(c->N, rp, ci, vals, c->fp32, ph, v, s, c->spmv_lanes, c->spmv_layout);
+        launch_multidot(n, 1, rh, 0, v, part, s);
         double d0;
-        CMG_TRY(multidot(c, 1, rh, 0, v, &d0));
+        CMG_TRY(host_sums(1, &d0));
         bool breakdown = !(std::fabs(d0) > 0.0) || !std::isfinite(d0);
         if (!breakdown) {
             alpha = rho / d0;
-            launch_waxpby(n, 1.0, r, -alpha, v, sv, s);   // s = r - alpha v
-            launch_block_jacobi_apply(c->N, c->dinv.as<double>(), sv, sh, s);
+            launch_bicg_s_bj(c->N, alpha, r, v, dinv, sv, sh, s);   // s = r - alpha v, sh = D^-1 s
             launch_spmv(c->N, rp, ci, vals, c->fp32, sh, t, s, c->spmv_lanes, c->spmv_layout);
-            launch_multidot(n, 3, rh, n, t, part, s);      // (rh,t) (s,t) (t,t)
-            launch_reduce_partials(nb, 3, part, dh, s);
-            launch_multidot(n, 2, rh, n, sv, part, s);     // (rh,s) (s,s)
-            launch_reduce_partials(nb, 2, part, dh + 3, s);
-            CMG_HIP_CHECK(hipMemcpyAsync(c->h_pinned, dh, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
-            CMG_HIP_CHECK(hipStreamSynchronize(s));
-            const double rt = c->h_pinned[0], st_ = c->h_pinned[1], tt = c->h_pinned[2];
-            const double rs = c->h_pinned[3], ss = c->h_pinned[4];
+            launch_bicg_dots5(n, rh, sv, t, part, s);              // (rh,t) (s,t) (t,t) (rh,s) (s,s)
+            double d5[5];
+            CMG_TRY(host_sums(5, d5));
+            const double rt = d5[0], st_ = d5[1], tt = d5[2], rs = d5[3], ss = d5[4];
             omega = tt > 0.0 ? st_ / tt : 0.0;
-            launch_axpby(n, alpha, ph, 1.0, x, s);        // x += alpha ph + omega sh
-            launch_axpby(n, omega, sh, 1.0, x, s);
-            launch_waxpby(n, 1.0, sv, -omega, t, r, s);   // r = s - omega t
+            launch_bicg_xr(n, alpha, omega, ph, sh, sv, t, x, r, s);   // x += alpha ph + omega sh, r = s - omega t
             it++;
             rn = std::sqrt(std::max(ss - 2.0 * omega * st_ + omega * omega * tt, 0.0));
             rho_new = rs - omega * rt;

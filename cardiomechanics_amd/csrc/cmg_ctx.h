@@ -98,6 +98,8 @@ struct cmg_ctx {
     cmg::DevBuf gm_V, gm_partial, gm_h, gm_y;
     int gm_m = 0;
     double* h_pinned = nullptr;   // host staging for reductions
+    double* h_part = nullptr;     // host staging for per-chunk dot partials (BiCGStab), 5 per chunk
+    int64_t h_part_len = 0;
     cmg::DevBuf corrupt;
     // static solver bookkeeping (CBSolverEquilibrium::UpdateInitialGuess)
     double time_last = 0.0, step_last = 0.0;

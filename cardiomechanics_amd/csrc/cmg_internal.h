@@ -119,6 +119,17 @@ int dot_blocks(int64_t n);
 void launch_multidot(int64_t n, int k, const double* V, int64_t ld, const double* w, double* partial, hipStream_t s);
 void launch_reduce_partials(int nblk, int k, const double* partial, double* out, hipStream_t s);
 void launch_multiaxpy(int64_t n, int k, const double* V, int64_t ld, const double* h, double* w, hipStream_t s);
+// fused BiCGStab updates: p = r + beta (p - omega v) (p = r when fresh) and ph = D^-1 p;
+// s = r - alpha v and sh = D^-1 s; x += alpha ph + omega sh and r = s - omega t;
+// partial[b][5] = (rh,t) (s,t) (t,t) (rh,s) (s,s) per dot chunk (dot_blocks(n) chunks)
+void launch_bicg_p_bj(int64_t N, bool fresh, double beta, double omega, const double* r, const double* v,
+                      const double* dinv, double* p, double* ph, hipStream_t s);
+void launch_bicg_s_bj(int64_t N, double alpha, const double* r, const double* v, const double* dinv, double* sv,
+                      double* sh, hipStream_t s);
+void launch_bicg_xr(int64_t n, double alpha, double omega, const double* ph, const double* sh, const double* sv,
+                    const double* t, double* x, double* r, hipStream_t s);
+void launch_bicg_dots5(int64_t n, const double* rh, const double* sv, const double* t, double* partial,
+                       hipStream_t s);
 void launch_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s);      // y = a x + b y
 void launch_waxpby(int64_t n, double a, const double* x, double b, const double* y, double* w, hipStream_t s);
 void launch_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s);

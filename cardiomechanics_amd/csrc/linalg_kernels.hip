@@ -257,6 +257,79 @@ __global__ __launch_bounds__(kDotThreads) void k_multidot(int64_t n, int k, cons
     }
 }
 
+// ---- fused BiCGStab vector updates (host-side scalars), one thread per node ----
+// p = r + beta (p - omega v) (p = r on a (re)start), then ph = D^-1 p
+__global__ __launch_bounds__(256) void k_bicg_p_bj(int64_t N, int fresh, double beta, double omega,
+                                                   const double* r, const double* v, const double* dinv, double* p,
+                                                   double* ph) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double q[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int64_t i = 3 * n + c;
+        q[c] = fresh ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+        p[i] = q[c];
+    }
+    const double* a = dinv + 9 * n;
+    ph[3 * n] = a[0] * q[0] + a[1] * q[1] + a[2] * q[2];
+    ph[3 * n + 1] = a[3] * q[0] + a[4] * q[1] + a[5] * q[2];
+    ph[3 * n + 2] = a[6] * q[0] + a[7] * q[1] + a[8] * q[2];
+}
+
+// s = r - alpha v, then sh = D^-1 s
+__global__ __launch_bounds__(256) void k_bicg_s_bj(int64_t N, double alpha, const double* r, const double* v,
+                                                   const double* dinv, double* sv, double* sh) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double q[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int64_t i = 3 * n + c;
+        q[c] = r[i] - alpha * v[i];
+        sv[i] = q[c];
+    }
+    const double* a = dinv + 9 * n;
+    sh[3 * n] = a[0] * q[0] + a[1] * q[1] + a[2] * q[2];
+    sh[3 * n + 1] = a[3] * q[0] + a[4] * q[1] + a[5] * q[2];
+    sh[3 * n + 2] = a[6] * q[0] + a[7] * q[1] + a[8] * q[2];
+}
+
+// x += alpha ph + omega sh;  r = s - omega t
+__global__ __launch_bounds__(256) void k_bicg_xr(int64_t n, double alpha, double omega, const double* ph,
+                                                 const double* sh, const double* sv, const double* t, double* x,
+                                                 double* r) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = x[i] + alpha * ph[i] + omega * sh[i];
+    r[i] = sv[i] - omega * t[i];
+}
+
+// partial[b][0..4] = (rh,t) (s,t) (t,t) (rh,s) (s,s) over the b-th chunk
+__global__ __launch_bounds__(kDotThreads) void k_bicg_dots5(int64_t n, const double* rh, const double* sv,
+                                                            const double* t, double* partial) {
+    __shared__ double sh[kDotThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kDotChunk;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kDotPerThread; q++) {
+        const int64_t i = base + (int64_t)q * kDotThreads + threadIdx.x;
+        if (i < n) {
+            const double a = rh[i], b = sv[i], c = t[i];
+            acc[0] += a * c;
+            acc[1] += b * c;
+            acc[2] += c * c;
+            acc[3] += a * b;
+            acc[4] += b * b;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        const double v = block_sum(acc[j], sh);
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 5 + j] = v;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_reduce_partials(int nblk, int k, const double* partial, double* out) {
     __shared__ double sh[4];
     const int j = blockIdx.x;
@@ -416,6 +489,31 @@ void launch_multidot(int64_t n, int k, const double* V, int64_t ld, const double
     const int nb = dot_blocks(n);
     if (nb == 0 || k == 0) return;
     hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kDotThreads), 0, s, n, k, V, ld, w, partial);
+}
+
+void launch_bicg_p_bj(int64_t N, bool fresh, double beta, double omega, const double* r, const double* v,
+                      const double* dinv, double* p, double* ph, hipStream_t s) {
+    if (N == 0) return;
+    hipLaunchKernelGGL(k_bicg_p_bj, grid1(N), dim3(256), 0, s, N, fresh ? 1 : 0, beta, omega, r, v, dinv, p, ph);
+}
+
+void launch_bicg_s_bj(int64_t N, double alpha, const double* r, const double* v, const double* dinv, double* sv,
+                      double* sh, hipStream_t s) {
+    if (N == 0) return;
+    hipLaunchKernelGGL(k_bicg_s_bj, grid1(N), dim3(256), 0, s, N, alpha, r, v, dinv, sv, sh);
+}
+
+void launch_bicg_xr(int64_t n, double alpha, double omega, const double* ph, const double* sh, const double* sv,
+                    const double* t, double* x, double* r, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_bicg_xr, grid1(n), dim3(256), 0, s, n, alpha, omega, ph, sh, sv, t, x, r);
+}
+
+void launch_bicg_dots5(int64_t n, const double* rh, const double* sv, const double* t, double* partial,
+                       hipStream_t s) {
+    const int nb = dot_blocks(n);
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_bicg_dots5, dim3(nb), dim3(kDotThreads), 0, s, n, rh, sv, t, partial);
 }
 
 void launch_reduce_partials(int nblk, int k, const double* partial, double* out, hipStream_t s) {
