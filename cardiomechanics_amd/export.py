@@ -55,11 +55,25 @@ def _data_array(name, arr, ncomp=1):
             f'format="binary">\n          {_b64(arr.ravel())}\n        </DataArray>\n')
 
 
+def cell_arrays(blocks):
+    """flat connectivity and offsets of cell blocks, each an (M_k, n_k) array of node ids"""
+    blocks = [np.asarray(b, np.int64) for b in blocks if len(b)]
+    if not blocks:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    conn = np.concatenate([b.ravel() for b in blocks])
+    offsets = np.cumsum(np.concatenate([np.full(b.shape[0], b.shape[1], np.int64) for b in blocks]))
+    return conn, offsets
+
+
 def write_vtu(path, points, cells, cell_types, point_data=None, cell_data=None):
-    """points (N,3); cells: list of connectivity rows (ragged allowed); cell_types (M,);
-    point_data / cell_data: {name: array (N,) / (N,k)}"""
-    conn = np.concatenate([np.asarray(c, np.int64).ravel() for c in cells]) if cells else np.zeros(0, np.int64)
-    offsets = np.cumsum([len(c) for c in cells]).astype(np.int64)
+    """points (N,3); cells: list of connectivity rows (ragged allowed) or a prepared
+    (connectivity, offsets) pair from cell_arrays; cell_types (M,); point_data /
+    cell_data: {name: array (N,) / (N,k)}"""
+    if isinstance(cells, tuple):
+        conn, offsets = cells
+    else:
+        conn = np.concatenate([np.asarray(c, np.int64).ravel() for c in cells]) if cells else np.zeros(0, np.int64)
+        offsets = np.cumsum([len(c) for c in cells]).astype(np.int64)
     out = ['<?xml version="1.0"?>\n',
            '<VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian" header_type="UInt64">\n',
            '  <UnstructuredGrid>\n',
@@ -147,12 +161,14 @@ class VTKExporter:
             fh.write('<VTKFile type="Collection" version="0.1" byte_order="LittleEndian">\n'
                      "\t<Collection>\n\t</Collection>\n</VTKFile>\n")
         el = mesh.elements
-        self._cells = [row for row in el]
-        self._types = [VTK_QUADRATIC_TETRA if el.shape[1] == 10 else VTK_TETRA] * len(el)
+        types = [np.full(len(el), VTK_QUADRATIC_TETRA if el.shape[1] == 10 else VTK_TETRA, np.uint8)]
+        blocks = [el]
         if mesh.surfaces.shape[0]:
             st = VTK_QUADRATIC_TRIANGLE if mesh.surfaces.shape[1] == 6 else VTK_TRIANGLE
-            self._cells += [row for row in mesh.surfaces]
-            self._types += [st] * mesh.surfaces.shape[0]
+            blocks.append(mesh.surfaces)
+            types.append(np.full(mesh.surfaces.shape[0], st, np.uint8))
+        self._cells = cell_arrays(blocks)
+        self._types = np.concatenate(types)
         self.n_solid = len(el)
 
     @property
