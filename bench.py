@@ -25,9 +25,10 @@ import numpy as np  # noqa: E402
 # f 240 + K_e 7200 (FP64) / 3600 (FP32 values)
 BYTES_PER_ELEM = {0: 1548 + 240 + 7200, 1: 1548 + 240 + 3600}
 FLOPS_PER_ELEM = 42_700
-# measured HBM bytes per element of the FP64 kernel: rocprofv3 FETCH_SIZE + WRITE_SIZE per
-# dispatch, calibrated on a copy kernel of known size (profiles/element_residual_tangent_lv197k_fp64.md)
-TRAFFIC_PER_ELEM = {0: 9_093, 1: None}
+# measured HBM bytes per element of the default FP64 kernel (variant 6): rocprofv3 FETCH_SIZE +
+# WRITE_SIZE per dispatch, calibrated on a copy kernel of known size
+# (profiles/element_residual_tangent_lv197k_fp64.md); variant 4 measured 9,093
+TRAFFIC_PER_ELEM = {0: 9_082, 1: None}
 HBM_PEAK = 8.0e12
 
 
@@ -129,7 +130,7 @@ def main():
 
     # kernel variants of the fused residual + tangent element kernel; the fastest is the headline
     variants = {}
-    for var in (2, 4, 3, 1):
+    for var in (2, 4, 5, 6, 3, 1):
         ctx = MechanicsContext(mesh, mats, precision=prec, element_variant=var)
         if xd is None:
             xd = DeviceArray(x.ravel())

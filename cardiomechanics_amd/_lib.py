@@ -82,7 +82,7 @@ class CmgStepStats(ctypes.Structure):
 class CmgBatchInfo(ctypes.Structure):
     _fields_ = [("n_sets", ctypes.c_int32), ("element_variant", ctypes.c_int32), ("spmv_lanes", ctypes.c_int32),
                 ("cg_chunk", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("tuned", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2), ("element_ms", ctypes.c_double * 4),
+                ("reserved", ctypes.c_int32 * 2), ("element_ms", ctypes.c_double * 6),
                 ("spmv_ms", ctypes.c_double * 3), ("tune_seconds", ctypes.c_double)]
 
     def as_dict(self):

@@ -140,7 +140,7 @@ void cmg_default_options(cmg_options* o) {
     o->krylov_maxit = 4000;
     o->krylov_method = 2;   // BiCGStab, host-side scalars: the fastest on the 200k-element LV beat
     o->device = 0;
-    o->element_variant = 0;   // by precision: 4 (FP64), 2 (mixed), the fastest of each on the LV
+    o->element_variant = 0;   // by precision: 6 (FP64), 2 (mixed), the fastest of each on the LV
     o->newton_atol = 1e-8;
     o->newton_rtol = 1e-8;
     o->newton_stol = 1e-8;
@@ -185,8 +185,8 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
         c->opt = *options;
     else
         cmg_default_options(&c->opt);
-    if (c->opt.element_variant < 1 || c->opt.element_variant > 4)
-        c->opt.element_variant = (c->opt.precision == CMG_PREC_MIXED) ? 2 : 4;
+    if (c->opt.element_variant < 1 || c->opt.element_variant > 6)
+        c->opt.element_variant = (c->opt.precision == CMG_PREC_MIXED) ? 2 : 6;
     c->device = c->opt.device;
     CMG_HIP_CHECK(hipSetDevice(c->device));
     CMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1246,7 +1246,7 @@ int residual_norm_at(cmg_ctx* c, double time, double cm, double* out) {
 // Newton on r(d) = f_int(nodes + d) + f_p(nodes + d) [+ cm M (d - d~)] = 0, the
 // SNES newtonls loop of CBSolverNewmarkBeta.cpp:890 / CBSolverEquilibrium.cpp:287 with
 // convergence on abstol / rtol / stol = Solver.Precision, a fresh exact tangent
-// per iteration, an inexact GMRES solve (Eisenstat-Walker forcing) and
+// per iteration, an inexact Krylov solve (Eisenstat-Walker forcing) and
 // backtracking on the residual norm.
 int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
     hipStream_t s = c->stream;

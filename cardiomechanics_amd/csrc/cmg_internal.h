@@ -74,7 +74,8 @@ struct SurfArgs {
 void launch_element_export(int npe, const ElemArgs& a, const double* Xref, double* out, hipStream_t s);
 void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s);
 // variant: 1 = per-lane quadrature points, 2 = shared stage (LDS), 3 = shared stage, two node passes,
-// 4 = shared stage with two lanes (two waves) per element and spatial pair
+// 4 = shared stage with two lanes (two waves) per element and spatial pair; 5 / 6 = variants 2 / 4
+// with two 7-element groups per workgroup, so the quadrature stage fills 56 lanes of the first wave
 void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s, int variant);
 void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s);
 void launch_t10_shape(int64_t E, const int32_t* conn_aos, const double* X, double* dNdX_soa, double* w, hipStream_t s);

@@ -544,20 +544,22 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
 // ---------------------------------------------------------------------------
 constexpr int kQRec = 51;   // Ft(9) BBt(9) S(9) Z(9) Y|X(9) czz codot ceq d00 d11 mfs
 
-template <int NPE, int NQP, typename TT, int NSPLIT, int BSPLIT = 1>
-__global__ __launch_bounds__(64 * BSPLIT) void k_residual_tangent_v2(const ElemArgs a0) {
+template <int NPE, int NQP, typename TT, int NSPLIT, int BSPLIT = 1, int GROUPS = 1>
+__global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(const ElemArgs a0) {
     constexpr int EPW = kTanElemsPerWave;
+    constexpr int EPG = EPW * GROUPS;     // elements per workgroup
+    static_assert(EPG * NQP <= 64, "stage A runs in the first wave");
     ElemArgs a;
     if (!batch_select(a0, a)) return;   // uniform per workgroup: no barrier below is skipped by part of it
-    __shared__ double sT[EPW][NQP][3 * NPE];
-    __shared__ TT sG[EPW][NQP][NPE * 3];
-    __shared__ TT sQ[EPW][NQP][kQRec];
+    __shared__ double sT[EPG][NQP][3 * NPE];
+    __shared__ TT sG[EPG][NQP][NPE * 3];
+    __shared__ TT sQ[EPG][NQP][kQRec];
     const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x;
     const int64_t E = a.E;
-    const int64_t e0 = (int64_t)bid * EPW;
+    const int64_t e0 = (int64_t)bid * EPG;
     // ---- stage A ----
-    if (lane < EPW * NQP) {
+    if (lane < EPG * NQP) {
         const int slot = lane / NQP, q = lane - NQP * (lane / NQP);
         const int64_t e = e0 + slot;
         if (e < E) {
@@ -623,12 +625,14 @@ __global__ __launch_bounds__(64 * BSPLIT) void k_residual_tangent_v2(const ElemA
     }
     __syncthreads();
     // ---- stage B ----
-    const int bl = lane % 64, bh = lane / 64;   // lane in its wave; part of the b range
-    const int slot = bl / 9;
-    const int ij = bl - 9 * slot;
+    // wave w serves element group w / BSPLIT and part w % BSPLIT of the b range
+    const int bl = lane % 64, bh = (lane / 64) % BSPLIT, grp = lane / (64 * BSPLIT);
+    const int lslot = bl / 9;
+    const int ij = bl - 9 * lslot;
     const int ii = ij / 3, jj = ij - 3 * (ij / 3);
+    const int slot = grp * EPW + lslot;
     const int64_t e = e0 + slot;
-    if (slot >= EPW || e >= E) return;
+    if (lslot >= EPW || e >= E) return;
     const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
     const bool guc = m.law == CMG_LAW_GUCCIONE;
     TT bmat[9];
@@ -1072,10 +1076,18 @@ void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s) {
 void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s, int variant) {
     if (a.E == 0) return;
     const int nblk = (int)((a.E + kTanElemsPerWave - 1) / kTanElemsPerWave);
+    const int nblk2 = (int)((a.E + 2 * kTanElemsPerWave - 1) / (2 * kTanElemsPerWave));
 #define CMG_LAUNCH(K) hipLaunchKernelGGL(K, dim3(nblk, a.n_sets), dim3(64), 0, s, a)
 #define CMG_LAUNCH2(K) hipLaunchKernelGGL(K, dim3(nblk, a.n_sets), dim3(128), 0, s, a)
+#define CMG_LAUNCH_G2(K, T) hipLaunchKernelGGL(K, dim3(nblk2, a.n_sets), dim3(T), 0, s, a)
     if (npe == 10) {
-        if (variant == 1) {
+        if (variant == 5) {
+            if (fp32) CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, float, 1, 1, 2>), 128);
+            else CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, double, 1, 1, 2>), 128);
+        } else if (variant == 6) {
+            if (fp32) CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, float, 1, 2, 2>), 256);
+            else CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, double, 1, 2, 2>), 256);
+        } else if (variant == 1) {
             if (fp32) CMG_LAUNCH((k_residual_tangent<10, 4, float>));
             else CMG_LAUNCH((k_residual_tangent<10, 4, double>));
         } else if (variant == 3) {
@@ -1099,6 +1111,7 @@ void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipS
     }
 #undef CMG_LAUNCH
 #undef CMG_LAUNCH2
+#undef CMG_LAUNCH_G2
 }
 
 void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s) {

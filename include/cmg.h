@@ -117,8 +117,9 @@ typedef struct cmg_options {
     int32_t device;             /* HIP device ordinal (default 0) */
     int32_t element_variant;    /* residual+tangent kernel: 2 shared quadrature stage, 3 the same in
                                    two node passes, 4 the same with two lanes per element and
-                                   spatial pair, 1 per-lane quadrature points; 0 (default): 4 in
-                                   FP64, 2 in mixed precision */
+                                   spatial pair, 1 per-lane quadrature points, 5 / 6 variants 2 / 4
+                                   with 14 elements per workgroup; 0 (default): 6 in FP64, 2 in
+                                   mixed precision */
     int32_t krylov_method;      /* 0 GMRES(restart), 1 BiCGStab with device-side scalars and graph-replayed
                                    iterations, 2 BiCGStab with host-side scalars (default); all right
                                    block-Jacobi preconditioned */
@@ -295,7 +296,7 @@ typedef struct cmg_batch_info {
     int32_t use_graph;
     int32_t tuned;             /* 1 once the autotuner has run */
     int32_t reserved[2];
-    double element_ms[4];      /* measured launch time of element variants 1..4 (0: not timed) */
+    double element_ms[6];      /* measured launch time of element variants 1..6 (0: not timed) */
     double spmv_ms[3];         /* measured batched SpMV time at 4, 8, 16 lanes */
     double tune_seconds;
 } cmg_batch_info;
@@ -305,7 +306,7 @@ typedef struct cmg_batch_info {
  * outlive it.  Every set starts from ctx's current node coordinates. */
 int cmg_batch_create(cmg_ctx* ctx, int32_t n_sets, const cmg_material* materials, cmg_batch** out);
 int cmg_batch_destroy(cmg_batch* batch);
-/* use_graph 0/1; spmv_lanes 0 (autotune) or 4/8/16; element_variant 0 (autotune) or 1..4;
+/* use_graph 0/1; spmv_lanes 0 (autotune) or 4/8/16; element_variant 0 (autotune) or 1..6;
  * cg_chunk >= 1.  Call before the first step. */
 int cmg_batch_configure(cmg_batch* batch, int32_t use_graph, int32_t spmv_lanes, int32_t element_variant,
                         int32_t cg_chunk);
