@@ -629,6 +629,7 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
     exportFile_ = p.Get<std::string>("Plugins.ApplyPressureFromFunction.ExportFile", "");
     appliedPressures_.assign(std::max<size_t>(1, groupIds_.size()), 0.0);
     if (circulation_.Active()) circulation_.InitVolumes(CavityVolumes(), timing_.GetTimeStep());
+    exporter_.Init(p, m, ctx_);
 }
 
 // Apply of every plugin before a solve (CBSolverNewmarkBeta.cpp:839-840): the
@@ -752,6 +753,7 @@ void CBSolverGPU::Run(std::vector<StepRecord>* records, int maxSteps) {
             first = false;
             timing_.ResetCounter();
         }
+        if (exporter_.Due(time)) exporter_.Export(time);   // CBSolver.cpp:1808-1818
         circulation_.WriteToFile(time);
         const bool exitNow = circulation_.ExitCheck(time);
         if (maxSteps > 0 && ++steps >= maxSteps) break;

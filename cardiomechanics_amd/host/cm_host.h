@@ -10,6 +10,7 @@
 //   CBApplyPressureFromFunction       prescribed cavity pressure (CBApplyPressureFromFunction.cpp:20-132)
 //   CBCirculation, CBCircWindkessel3  cavity pressure from a 0D circulation (CBCirculation.cpp,
 //                                     CBCircWindkessel3.cpp, CBCircModel.hpp)
+//   CBModelExporterVTK                .pvd / .vtu result export (CBModelExporterVTK.cpp)
 //   CBSolverGPU                       Init / SolverStep / Run (CBSolver.h:68-215) with the static
 //                                     (CBSolverEquilibrium) and Newmark (CBSolverNewmarkBeta) steps
 #pragma once
@@ -267,6 +268,36 @@ private:
 };
 
 // ---------------------------------------------------------------------------
+// CBModelExporterVTK (CBModel/CBModelExporterVTK.cpp:29-499) with the fields of
+// CBSolver::Export (CBSolver.cpp:538-1164) and CBSolverNewmarkBeta::Export
+// (:800-827): <dir>/<prefix>.pvd and <dir>/<prefix>_vtu/<prefix>.<n>.vtu, binary
+// base64 arrays with UInt64 headers; Export.Prefix, Export.TimeStep, Export.Options.*.
+// The same files as cardiomechanics_amd/export.py writes.
+class CBModelExporterVTK {
+public:
+    void Init(const ParameterMap& p, const TetgenModel& m, cmg_ctx* ctx);
+    bool Active() const { return active_; }
+    bool Due(double time) const;     // the start, then every Export.TimeStep (CBSolver.cpp:1808-1810)
+    void Export(double time);
+    int Counter() const { return counter_; }
+
+private:
+    bool Option(const std::string& name) const;
+    bool active_ = false;
+    cmg_ctx* ctx_ = nullptr;
+    std::string dir_, prefix_;
+    double timeStep_ = 1e-3, lastTime_ = 0.0;
+    bool exported_ = false;
+    int counter_ = 0;
+    std::map<std::string, bool> options_;
+    int64_t nNodes_ = 0, nSolid_ = 0;
+    std::vector<double> refCoords_, basis0_;    // reference coordinates, centroid fibre frames [E][9]
+    std::vector<int32_t> material_, fixation_;
+    std::vector<int64_t> connectivity_, offsets_;
+    std::vector<uint8_t> types_;
+};
+
+// ---------------------------------------------------------------------------
 class CBSolverGPU;
 
 class CBFormulationTotalLagrangianGPU {
@@ -315,6 +346,7 @@ private:
     CBTiming timing_;
     CBApplyPressureFromFunction pressure_;
     CBCirculation circulation_;
+    CBModelExporterVTK exporter_;
     std::vector<double> appliedPressures_;
     void ApplyPlugins(double time);
     std::unique_ptr<CBFormulationTotalLagrangianGPU> formulation_;

@@ -287,6 +287,7 @@ def test_host_cpp_circulation_matches_python(tmp_path):
     subprocess.check_call(["g++", "-O1", "-std=c++17", "-I", os.path.join(root, "include"), "-I", host,
                            os.path.join(root, "tests", "host", "circ_linear.cpp"),
                            os.path.join(host, "cm_circulation.cpp"), os.path.join(host, "cm_host.cpp"),
+                           os.path.join(host, "cm_export.cpp"),
                            "-L", pkg, "-lcmg", f"-Wl,-rpath,{pkg}", "-o", str(exe)])
     for mode in ({}, {"Plugins.Circulation.Perturbation": False, "Plugins.Circulation.SecantIterations": 0}):
         params = wk3_params(**mode)

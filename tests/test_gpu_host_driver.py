@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from cardiomechanics_amd import circulation as CI
+from cardiomechanics_amd import export as X
 from cardiomechanics_amd import materials as MT
 from cardiomechanics_amd import mesh as M
 from cardiomechanics_amd import settings as ST
@@ -28,7 +29,9 @@ def test_cmg_driver_cube_static_matches_python_host(tmp_path):
     M.write_tetgen(str(tmp_path / "cube"), mesh)
     shutil.copy(os.path.join(ROOT, "cardiomechanics_amd", "configs", "cube_static.xml"), tmp_path / "cube.xml")
     out = subprocess.run([DRIVER, "cube.xml", "-parameter", "Solver.StopTime=0.3", "-parameter",
-                          "Solver.Precision=1e-11"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+                          "Solver.Precision=1e-11", "-parameter", "Export.Prefix=results/cube", "-parameter",
+                          "Export.TimeStep=0.1", "-parameter", "Export.Options.Jacobian=true"],
+                         cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(out.stdout[-2000:], out.stderr[-2000:])
     assert out.returncode == 0
     vols = [float(v) for v in re.findall(r"volume=([-0-9.e+]+)", out.stdout)]
@@ -37,9 +40,22 @@ def test_cmg_driver_cube_static_matches_python_host(tmp_path):
     with MechanicsContext(mesh, [MT.cube_guccione()], newton_atol=1e-11, newton_rtol=1e-11,
                           newton_stol=1e-11) as ctx:
         recs = SV.run(ctx, "static", SV.Timing(0.0, 0.3, 0.1), SV.linear_ramp(2000.0, 0.0, 1.0))
+        ex = X.VTKExporter(ctx, mesh, {"Export.Prefix": str(tmp_path / "py" / "cube"),
+                                       "Export.Options.Jacobian": "true"})
+        py = X.read_vtu(ex.export(0.3))
     ref = [r.volumes[0] for r in recs]
     np.testing.assert_allclose(vols, ref, rtol=1e-9)
     assert os.path.exists(tmp_path / "cube_pressure.dat")
+    # the C++ exporter (CBModelExporterVTK) wrote the same fields as the Python one
+    pvd = open(tmp_path / "results" / "cube.pvd").read()
+    assert pvd.count("<DataSet") == 4
+    cpp = X.read_vtu(str(tmp_path / "results" / "cube_vtu" / "cube.3.vtu"))
+    assert np.array_equal(cpp["connectivity"], py["connectivity"]) and np.array_equal(cpp["types"], py["types"])
+    np.testing.assert_allclose(cpp["points"], py["points"], rtol=1e-10, atol=1e-14)
+    assert set(cpp["cell_data"]) == set(py["cell_data"]) and set(cpp["point_data"]) == set(py["point_data"])
+    for k in ("J", "Lambda", "f", "S_ii", "S_ij", "E_ii"):
+        a, b = cpp["cell_data"][k], py["cell_data"][k]
+        np.testing.assert_allclose(a, b, rtol=1e-7, atol=1e-9 * np.abs(b).max(), err_msg=k)
 
 
 def test_cmg_driver_lv_windkessel_matches_python_host(tmp_path, monkeypatch):

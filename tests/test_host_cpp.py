@@ -16,6 +16,7 @@ def test_host_layer_selftest(tmp_path):
     subprocess.check_call(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I",
                            os.path.join(PKG, "host"), os.path.join(ROOT, "tests", "host", "host_selftest.cpp"),
                            os.path.join(PKG, "host", "cm_host.cpp"), os.path.join(PKG, "host", "cm_circulation.cpp"),
+                           os.path.join(PKG, "host", "cm_export.cpp"),
                            "-L", PKG, "-lcmg",
                            f"-Wl,-rpath,{PKG}", "-o", str(exe)])
     out = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True)
