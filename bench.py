@@ -275,9 +275,12 @@ def beat_full(args):
         ctx = MechanicsContext(mesh, mats, precision=prec)
         states, stats = {}, []
 
-        def on(rec, st, ctx=ctx, states=states, stats=stats):
+        def on(rec, st, ctx=ctx, states=states, stats=stats, label=label):
             states[round(rec.time * 1e6)] = ctx.get_state()["nodes"] - X0
             stats.append((rec.dt, st.newton_iterations, st.krylov_iterations))
+            if len(stats) % 50 == 0:
+                log(f"[beat] {label} t={rec.time:.4f} s dt={rec.dt:.2e} newton={st.newton_iterations} "
+                    f"krylov={st.krylov_iterations}")
 
         t0 = time.perf_counter()
         SV.run(ctx, "newmark", SV.Timing(0.0, args.beat_end, 1e-3), SV.sinus_pulse(-16000.0, 0.0, 0.8),

@@ -1225,6 +1225,9 @@ namespace {
 // smallest backtracking factor when a Newton step inverts an element (the reference's
 // line search goes down to 1e-12 on a domain error)
 constexpr double kMinDomainLambda = 1.0 / (1 << 20);
+// SNES divergence tolerance (PETSc's default divtol): ||r|| > 1e4 ||r_0|| ends the
+// step as diverged (SNES_DIVERGED_DTOL)
+constexpr double kNewtonDivTol = 1e4;
 
 // ||r(d)|| of the Newmark residual at the current increment d (infinite when an
 // element inverts)
@@ -1323,11 +1326,18 @@ int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
             st->converged_reason = 2;
             break;
         }
+        if (it > 0 && rnorm > kNewtonDivTol * r0) {
+            st->converged_reason = -9;   // SNES_DIVERGED_DTOL
+            break;
+        }
         if (it > 0 && rnorm < c->opt.newton_rtol * r0) {
             st->converged_reason = 3;
             break;
         }
-        if (it > 0 && snorm * lambda < c->opt.newton_stol * dnorm) {
+        // a short step ends Newton only after a residual decrease: the reference's
+        // backtracking line search accepts no step that raises ||r||, where the
+        // loop above gives up backtracking at 1/64 and carries on
+        if (it > 0 && rnorm <= rprev && snorm * lambda < c->opt.newton_stol * dnorm) {
             st->converged_reason = 4;
             break;
         }
@@ -1351,6 +1361,12 @@ int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
         launch_axpby(n, 1.0, delta, 1.0, d, s);
         CMG_TRY(norm2(c, delta, &snorm));
         CMG_TRY(norm2(c, d, &dnorm));
+        if (!std::isfinite(ls.final_residual) || !(snorm > 0.0) || !std::isfinite(snorm)) {
+            // the Krylov solve broke down without a usable direction (a zero or
+            // non-finite update): SNES_DIVERGED_LINEAR_SOLVE, never a short step
+            st->converged_reason = -3;
+            break;
+        }
     }
     st->newton_iterations = it;
     st->krylov_iterations = kry_total;

@@ -258,32 +258,109 @@ LV_EPI = (10e-3, 20e-3)      # short / long semi-axis of the epicardium
 LV_BASE_Z = 5e-3             # truncation plane in the ellipsoid frame
 
 
+def lv_ring_counts(ns: int, nv: int) -> np.ndarray:
+    """Vertices of rings 0..ns of the LV surface grid: the apex (one vertex), then 6j
+    on ring j (rings near the pole are nearly circles of radius proportional to j, so
+    6j keeps their triangles close to equilateral) up to nv."""
+    return np.array([1] + [min(nv, 6 * j) for j in range(1, ns + 1)], np.int64)
+
+
+def lv_tet_count(nt: int, ns: int, nv: int) -> int:
+    """Tetrahedra of lv_mesh(dims=(nt, ns, nv)): 3 per prism, one prism per surface
+    triangle and layer; the strip between rings of n and m vertices holds n + m
+    triangles, the apex fan n_1."""
+    n = lv_ring_counts(ns, nv)
+    tris = int(n[1] + np.sum(n[1:-1] + n[2:]))
+    return 3 * nt * tris
+
+
 def lv_dims_for(n_target: int):
-    """(nt, ns, nv) hex counts giving about n_target tetrahedra (6 per hex)."""
-    hexes = n_target / 6.0
-    # aspect: thickness 3 mm, meridian ~ 30 mm, circumference ~ 55 mm
-    c = (hexes / (1.0 * 10.0 * 18.3)) ** (1.0 / 3.0)
+    """(nt, ns, nv) giving about n_target tetrahedra: layer and ring counts from the
+    wall's aspect (thickness 3 mm, meridian ~30 mm, circumference ~55 mm), then the
+    largest ring size nv that comes closest to n_target."""
+    c = (n_target / 6.0 / (1.0 * 10.0 * 18.3)) ** (1.0 / 3.0)
     nt = max(2, int(round(1.0 * c)))
     ns = max(4, int(round(10.0 * c)))
-    nv = max(8, int(round(hexes / (nt * ns))))
-    return nt, ns, nv
+    nv0 = max(8, int(round(n_target / 6.0 / (nt * ns))))
+    best = min(range(8, 4 * nv0 + 8), key=lambda nv: abs(lv_tet_count(nt, ns, nv) - n_target))
+    return nt, ns, best
+
+
+def ring_strip_triangles(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Triangulate the closed strip between two rings of vertex ids a and b, both
+    ordered by angle and starting at angle 0 (vertex k of a ring of n at 2 pi k / n):
+    advance along whichever ring's next vertex comes first.  A one-vertex ring a
+    gives the fan around it."""
+    na, nb = len(a), len(b)
+    if na == 1:
+        k = np.arange(nb)
+        return np.stack([np.full(nb, a[0]), b[k], b[(k + 1) % nb]], axis=-1)
+    tris = []
+    i = j = 0
+    while i < na or j < nb:
+        if j >= nb or (i < na and (i + 1) * nb <= (j + 1) * na):
+            tris.append((a[i % na], a[(i + 1) % na], b[j % nb]))
+            i += 1
+        else:
+            tris.append((a[i % na], b[(j + 1) % nb], b[j % nb]))
+            j += 1
+    return np.array(tris, np.int64)
+
+
+def split_prisms(prisms: np.ndarray) -> np.ndarray:
+    """Split triangular prisms (bottom v0 v1 v2, top v3 v4 v5, v3 above v0, ...) into
+    3 tetrahedra each so that neighbouring prisms conform: every quadrilateral face is
+    cut by the diagonal through its smallest global vertex id (Dompierre, Labbe,
+    Vallet, Camarero, "How to subdivide pyramids, prisms and hexahedra into
+    tetrahedra", IMR 1999).  The prism is first turned (and flipped top to bottom) so
+    that its smallest vertex is v0; the faces at v0 then take their diagonals from v0
+    and the third face's diagonal decides between the two splits.  Orientation is left
+    to orient_t4."""
+    P = np.asarray(prisms, np.int64).copy()
+    amin = np.argmin(P, axis=1)
+    top = amin >= 3
+    P[top] = P[top][:, [3, 4, 5, 0, 1, 2]]
+    amin = np.where(top, amin - 3, amin)
+    for r, perm in ((1, [1, 2, 0, 4, 5, 3]), (2, [2, 0, 1, 5, 3, 4])):
+        P[amin == r] = P[amin == r][:, perm]
+    v0, v1, v2, v3, v4, v5 = P.T
+    cut15 = np.minimum(v1, v5) < np.minimum(v2, v4)
+    a = np.stack([np.stack([v0, v1, v2, v5], -1), np.stack([v0, v1, v5, v4], -1),
+                  np.stack([v0, v4, v5, v3], -1)], axis=1)
+    b = np.stack([np.stack([v0, v1, v2, v4], -1), np.stack([v0, v4, v2, v5], -1),
+                  np.stack([v0, v4, v5, v3], -1)], axis=1)
+    return np.where(cut15[:, None, None], a, b).reshape(-1, 4)
+
+
+def tet_quality(coords: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    """6 sqrt(2) |V| / l_rms^3 of each tetrahedron (1 for the regular one, 0 for a flat
+    one), l_rms the root mean square of its six edge lengths."""
+    p = coords[tets[:, :4]]
+    vol = np.abs(signed_volume6(coords, tets)) / 6.0
+    l2 = sum(np.sum((p[:, a] - p[:, b]) ** 2, axis=1) for a, b in
+             ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))) / 6.0
+    return 6.0 * np.sqrt(2.0) * vol / l2 ** 1.5
 
 
 def lv_mesh(n_target: int = 200_000, dims=None, order: int = 10, surface_type: int = 3,
             fibre_deg=(60.0, -60.0), activation=(0.0, 0.08), endo_normal: str = "cavity") -> Mesh:
     """Idealised truncated prolate-ellipsoid LV (Land et al. 2015 benchmark shape).
 
-    The wall is a structured (transmural t, meridian s, circumferential v) grid:
-    r_s(t) = 7+3t mm, r_l(t) = 17+3t mm, u(s) from -pi (apex) to -acos(5 mm / r_l(t)),
-    so every layer ends on the plane z = 5 mm.  Coordinates are shifted by -5 mm so
-    the base lies in z = 0; the cavity volume taken about the origin
-    (CBCirculationCavity.cpp:77-94) is then the volume closed by the base plane.
-    Hexes are split into 6 Kuhn tetrahedra; the apex layer uses prisms split in 3.
-    Base nodes are clamped (bc=7).  Surface 1 = endocardium, oriented like the
-    reference's ellipsoid.sur: normal out of the wall, into the cavity.  The
-    residual carries the pressure forces of CBElementSurfaceT3::CalcForcesDueToPressure
-    (-p A N n), so a negative pressure amplitude inflates (as in the reference's
-    benchmark Problem2.xml) and the cavity volume sum_i x_0.(a x b)/6 is negative.  Fibres: helix angle linear in t from
+    Wall surfaces of transmural parameter t: r_s(t) = 7+3t mm, r_l(t) = 17+3t mm,
+    u(s) from -pi (apex) to -acos(5 mm / r_l(t)), so every layer ends on the plane
+    z = 5 mm.  Coordinates are shifted by -5 mm so the base lies in z = 0; the cavity
+    volume taken about the origin (CBCirculationCavity.cpp:77-94) is then the volume
+    closed by the base plane.  dims = (nt, ns, nv): nt layers through the wall; on
+    each surface, rings j = 0..ns at s = j/ns, ring j holding min(6j, nv) vertices
+    (a polar cap with near-equilateral triangles, where a uniform ring size would make
+    needle-shaped elements at the apex).  The surface triangles (ring strips) are
+    extruded through the wall into prisms, each split into 3 tetrahedra by the
+    smallest-vertex rule, which keeps the mesh conforming.  Base nodes are clamped
+    (bc=7).  Surface 1 = endocardium, oriented like the reference's ellipsoid.sur:
+    normal out of the wall, into the cavity.  The residual carries the pressure forces
+    of CBElementSurfaceT3::CalcForcesDueToPressure (-p A N n), so a negative pressure
+    amplitude inflates (as in the reference's benchmark Problem2.xml) and the cavity
+    volume sum_i x_0.(a x b)/6 is negative.  Fibres: helix angle linear in t from
     fibre_deg[0] (endocardium) to fibre_deg[1] (epicardium), sheet = transmural.
     Activation rises linearly apex -> base over `activation` seconds.
     endo_normal="wall" turns the endocardium the other way (normal out of the blood
@@ -293,42 +370,29 @@ def lv_mesh(n_target: int = 200_000, dims=None, order: int = 10, surface_type: i
     if endo_normal not in ("cavity", "wall"):
         raise ValueError("endo_normal must be 'cavity' or 'wall'")
     nt, ns, nv = dims if dims is not None else lv_dims_for(n_target)
+    if nt < 1 or ns < 1 or nv < 6:
+        raise ValueError("lv_mesh needs nt >= 1, ns >= 1, nv >= 6")
     t = np.linspace(0.0, 1.0, nt + 1)
     rs = LV_ENDO[0] + (LV_EPI[0] - LV_ENDO[0]) * t
     rl = LV_ENDO[1] + (LV_EPI[1] - LV_ENDO[1]) * t
     uend = -np.arccos(LV_BASE_Z / rl)
-    s = np.linspace(0.0, 1.0, ns + 1)[1:]                      # rings 1..ns (ring 0 is the apex)
-    v = np.arange(nv) * (2 * np.pi / nv)
-    T, S, V = np.meshgrid(np.arange(nt + 1), s, v, indexing="ij")
-    U = -np.pi + S * (uend[T] + np.pi)
-    RS, RL = rs[T], rl[T]
-    ring = np.stack([RS * np.sin(U) * np.cos(V), RS * np.sin(U) * np.sin(V), RL * np.cos(U)], axis=-1)
-    apex = np.stack([np.zeros(nt + 1), np.zeros(nt + 1), -rl], axis=-1)
-    coords = np.concatenate([apex, ring.reshape(-1, 3)], axis=0)
+    counts = lv_ring_counts(ns, nv)
+    first = np.concatenate([[0], np.cumsum(counts)])
+    nsurf = int(first[-1])
+    s_v = np.concatenate([np.full(c, j / ns) for j, c in enumerate(counts)])
+    ang_v = np.concatenate([np.arange(c) * (2.0 * np.pi / c) for c in counts])
+    tris = np.concatenate([ring_strip_triangles(np.arange(first[j], first[j + 1]),
+                                                np.arange(first[j + 1], first[j + 2])) for j in range(ns)])
+    # vertex (layer i, surface vertex k) has id i * nsurf + k
+    U = -np.pi + s_v[None, :] * (uend[:, None] + np.pi)
+    RS, RL, V = rs[:, None], rl[:, None], ang_v[None, :]
+    coords = np.stack([RS * np.sin(U) * np.cos(V), RS * np.sin(U) * np.sin(V),
+                       np.broadcast_to(RL * np.cos(U), U.shape)], axis=-1).reshape(-1, 3)
     coords[:, 2] -= LV_BASE_Z
-    tcoord_v = np.concatenate([t, t[T].ravel()])              # transmural parameter of vertices
-    n_apex = nt + 1
-
-    def rid(i, j, k):   # ring vertex id, j = 1..ns, k periodic
-        return n_apex + (i * ns + (j - 1)) * nv + (k % nv)
-
-    # hexes between rings j and j+1 (j = 1..ns-1)
-    if ns > 1:
-        ii, jj, kk = np.meshgrid(np.arange(nt), np.arange(1, ns), np.arange(nv), indexing="ij")
-        ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
-        hex_tets = kuhn_tets(lambda d: rid(ii + d[0], jj + d[1], kk + d[2]))
-    else:
-        hex_tets = np.zeros((0, 4), np.int64)
-    # apex prisms: pole line (i, i+1) and ring-1 nodes k, k+1
-    ii, kk = np.meshgrid(np.arange(nt), np.arange(nv), indexing="ij")
-    ii, kk = ii.ravel(), kk.ravel()
-    P, Pp = ii, ii + 1
-    A, B = rid(ii, 1, kk), rid(ii, 1, kk + 1)
-    Ap, Bp = rid(ii + 1, 1, kk), rid(ii + 1, 1, kk + 1)
-    prism_tets = np.concatenate([np.stack([P, A, B, Bp], -1), np.stack([P, A, Ap, Bp], -1),
-                                 np.stack([P, Ap, Bp, Pp], -1)], axis=0)
-    tets = np.concatenate([prism_tets, hex_tets], axis=0).astype(np.int64)
-    tets = orient_t4(coords, tets)
+    tcoord_v = np.repeat(t, nsurf)                              # transmural parameter of vertices
+    off = np.repeat(np.arange(nt, dtype=np.int64) * nsurf, len(tris))[:, None]
+    tri = np.tile(tris, (nt, 1))
+    tets = orient_t4(coords, split_prisms(np.concatenate([tri + off, tri + off + nsurf], axis=1)))
     faces = orient_faces_outward(coords, boundary_faces(tets), tets)
     is_endo = np.all(tcoord_v[faces] == 0.0, axis=1)
     endo = faces[is_endo]                                      # normal out of the wall, into the cavity

@@ -143,7 +143,10 @@ typedef struct cmg_solve_stats {
 typedef struct cmg_step_stats {
     int32_t newton_iterations;   /* SNESGetIterationNumber */
     int32_t krylov_iterations;   /* KSPGetTotalIterations */
-    int32_t converged_reason;    /* >0 converged (2 atol, 3 rtol, 4 stol), <0 diverged */
+    int32_t converged_reason;    /* >0 converged (2 atol, 3 rtol, 4 stol), <0 diverged (-3 the Krylov
+                                    solve gave no usable update, -4 non-finite residual, -5
+                                    Solver.MaxIterations, -6 an element stays inverted, -9
+                                    ||r|| > 1e4 ||r_0||) */
     int32_t corrupt_element;     /* first element with det F <= 0, or -1 */
     double residual_norm;        /* final ||r||_2 */
     double initial_residual_norm;
