@@ -76,6 +76,10 @@ def main():
                          "precision and compare their displacements at every common time")
     ap.add_argument("--beat-elements", type=int, default=20_000)
     ap.add_argument("--beat-end", type=float, default=0.8)
+    ap.add_argument("--beat-precisions", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--beat-newmark", default="0.25,0.5", help="Newmark beta,gamma of the whole-beat runs")
+    ap.add_argument("--beat-rayleigh", default="0,0",
+                    help="Rayleigh damping alpha (1/s), beta (s); the reference's LV settings use 500,0.01")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU: sharding the mesh over GPUs (element "
@@ -259,9 +263,9 @@ def main():
 def beat_full(args):
     """Whole LV beats (Newmark, dt = 1 ms, CBTiming halving on failed steps, cavity pressure
     -16 kPa sin(pi t / 0.8 s), active tension 60 kPa sin^2 over 0.3 s from the apex-to-base
-    activation) in FP64 and in mixed precision at the reference's Solver.Precision; the
-    displacement difference at every time both runs accepted, relative to the FP64 run's
-    largest displacement over the beat."""
+    activation) in FP64 and/or mixed precision at the reference's Solver.Precision; with
+    both, the displacement difference at every time both runs accepted, relative to the
+    FP64 run's largest displacement over the beat."""
     from cardiomechanics_amd import materials as MT
     from cardiomechanics_amd import mesh as M
     from cardiomechanics_amd import solver as SV
@@ -271,12 +275,23 @@ def beat_full(args):
     mats = [MT.lv_holzapfel()]
     X0 = mesh.coords.ravel()
     runs = {}
+    labels = [x.strip() for x in args.beat_precisions.split(",") if x.strip()]
+    if not labels or any(x not in ("fp64", "mixed") for x in labels):
+        raise SystemExit("--beat-precisions takes fp64 and/or mixed")
+    alpha, beta_r = (float(x) for x in args.beat_rayleigh.split(","))
+    nm_beta, nm_gamma = (float(x) for x in args.beat_newmark.split(","))
     for label, prec in (("fp64", 0), ("mixed", 1)):
-        ctx = MechanicsContext(mesh, mats, precision=prec)
-        states, stats = {}, []
+        if label not in labels:
+            continue
+        ctx = MechanicsContext(mesh, mats, precision=prec, rayleigh_alpha=alpha, rayleigh_beta=beta_r,
+                               newmark_beta=nm_beta, newmark_gamma=nm_gamma)
+        states, stats, umax = {}, [], [0.0]
 
-        def on(rec, st, ctx=ctx, states=states, stats=stats, label=label):
-            states[round(rec.time * 1e6)] = ctx.get_state()["nodes"] - X0
+        def on(rec, st, ctx=ctx, states=states, stats=stats, label=label, umax=umax):
+            u = ctx.get_state()["nodes"] - X0
+            umax[0] = max(umax[0], float(np.abs(u).max()))
+            if len(labels) == 2:                     # kept only for the comparison
+                states[round(rec.time * 1e6)] = u
             stats.append((rec.dt, st.newton_iterations, st.krylov_iterations))
             if len(stats) % 50 == 0:
                 log(f"[beat] {label} t={rec.time:.4f} s dt={rec.dt:.2e} newton={st.newton_iterations} "
@@ -287,18 +302,24 @@ def beat_full(args):
                on_step=on)
         wall = time.perf_counter() - t0
         ctx.close()
-        runs[label] = (states, stats, wall)
+        runs[label] = (states, stats, wall, umax[0])
         log(f"[beat] {label}: {len(stats)} accepted steps ({sum(1 for s in stats if s[0] < 0.999e-3)} shorter "
             f"than 1 ms), newton {sum(s[1] for s in stats)}, krylov {sum(s[2] for s in stats)}, {wall:.1f} s")
-    ref, mix = runs["fp64"][0], runs["mixed"][0]
-    common = sorted(set(ref) & set(mix))
-    umax = max(float(np.abs(v).max()) for v in ref.values())
-    diff = max(float(np.abs(mix[k] - ref[k]).max()) for k in common) / umax
+    umax = runs[labels[0]][3]
+    common, diff = [], None
+    if len(runs) == 2:
+        ref, mix = runs["fp64"][0], runs["mixed"][0]
+        common = sorted(set(ref) & set(mix))
+        diff = max(float(np.abs(mix[k] - ref[k]).max()) for k in common) / umax
+    last = labels[-1]
     out = {"metric": "Newmark beat wall time", "unit": "s", "higher_is_better": False, "n_gpus": 1,
            "config": {"workload": "LV beat, Holzapfel-Ogden + active tension, Newmark dt 1 ms with halving",
-                      "elements": mesh.n_elements, "dofs": 3 * mesh.n_nodes, "end_time_s": args.beat_end},
-           "value": runs["mixed"][2], "dtype": "mixed (fp32 tangent and Krylov operator, fp64 residual)",
-           "fp64_wall_s": runs["fp64"][2], "mixed_wall_s": runs["mixed"][2],
+                      "elements": mesh.n_elements, "dofs": 3 * mesh.n_nodes, "end_time_s": args.beat_end,
+                      "rayleigh_alpha": alpha, "rayleigh_beta": beta_r, "newmark_beta": nm_beta,
+                      "newmark_gamma": nm_gamma},
+           "value": runs[last][2],
+           "dtype": "mixed (fp32 tangent and Krylov operator, fp64 residual)" if last == "mixed" else "f64",
+           "wall_s": {k: v[2] for k, v in runs.items()},
            "accepted_steps": {k: len(v[1]) for k, v in runs.items()},
            "newton": {k: sum(s[1] for s in v[1]) for k, v in runs.items()},
            "krylov": {k: sum(s[2] for s in v[1]) for k, v in runs.items()},

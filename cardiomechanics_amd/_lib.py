@@ -58,7 +58,8 @@ class CmgOptions(ctypes.Structure):
                 ("newton_atol", ctypes.c_double), ("newton_rtol", ctypes.c_double),
                 ("newton_stol", ctypes.c_double), ("krylov_rtol", ctypes.c_double),
                 ("newmark_beta", ctypes.c_double), ("newmark_gamma", ctypes.c_double),
-                ("fd_epsilon", ctypes.c_double), ("reserved", ctypes.c_double * 4)]
+                ("fd_epsilon", ctypes.c_double), ("rayleigh_alpha", ctypes.c_double),
+                ("rayleigh_beta", ctypes.c_double), ("reserved", ctypes.c_double * 2)]
 
 
 class CmgSolveStats(ctypes.Structure):

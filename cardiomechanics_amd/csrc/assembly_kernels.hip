@@ -105,7 +105,51 @@ __global__ __launch_bounds__(256) void k_gather_matrix(int64_t nnzb, const int64
     vals[t] = (VT)sum;
 }
 
+// C = beta_R K + alpha M (Rayleigh damping) in place over K's gathered BSR values, with
+// the rows and columns of constrained DOFs zeroed (their velocities are zero and their
+// residual rows are cleared, so these entries never act); beta_R = 0 ignores vals' old
+// contents
+__global__ __launch_bounds__(256) void k_rayleigh_finish(int64_t nnzb, const int32_t* blkrow, const int32_t* colind,
+                                                         const uint8_t* bcmask, const double* mval, double beta_r,
+                                                         double alpha, double* vals) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 9 * nnzb) return;
+    const int64_t slot = t / 9;
+    const int k = (int)(t - 9 * slot);
+    const int i = k / 3, j = k - 3 * (k / 3);
+    double v = beta_r != 0.0 ? beta_r * vals[t] : 0.0;
+    if (i == j) v += alpha * mval[slot];
+    const uint8_t mr = bcmask ? bcmask[blkrow[slot]] : 0, mc = bcmask ? bcmask[colind[slot]] : 0;
+    if (((mr >> i) & 1) || ((mc >> j) & 1)) v = 0.0;
+    vals[t] = v;
+}
+
+// vals += a * C over the BSR values (the damping part of the Newmark tangent)
+template <typename VT>
+__global__ __launch_bounds__(256) void k_add_scaled_values(int64_t nv, double a, const double* C, VT* vals) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    vals[t] = (VT)((double)vals[t] + a * C[t]);
+}
+
 }  // namespace
+
+void launch_rayleigh_finish(int64_t nnzb, const int32_t* blkrow, const int32_t* colind, const uint8_t* bcmask,
+                            const double* mval, double beta_r, double alpha, double* vals, hipStream_t s) {
+    if (nnzb == 0) return;
+    hipLaunchKernelGGL(k_rayleigh_finish, dim3((unsigned)((9 * nnzb + 255) / 256)), dim3(256), 0, s, nnzb, blkrow,
+                       colind, bcmask, mval, beta_r, alpha, vals);
+}
+
+void launch_add_scaled_values(int64_t nnzb, double a, const double* C, void* vals, bool vals_fp32, hipStream_t s) {
+    if (nnzb == 0) return;
+    const int64_t nv = 9 * nnzb;
+    const dim3 grid((unsigned)((nv + 255) / 256));
+    if (vals_fp32)
+        hipLaunchKernelGGL(k_add_scaled_values<float>, grid, dim3(256), 0, s, nv, a, C, (float*)vals);
+    else
+        hipLaunchKernelGGL(k_add_scaled_values<double>, grid, dim3(256), 0, s, nv, a, C, (double*)vals);
+}
 
 void launch_gather_vector(int64_t N, const int64_t* inc_ptr, const int32_t* inc, const double* fe, int,
                           const int64_t* sinc_ptr, const int32_t* sinc, const double* fs, int,

@@ -187,6 +187,8 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
         cmg_default_options(&c->opt);
     if (c->opt.element_variant < 1 || c->opt.element_variant > 6)
         c->opt.element_variant = (c->opt.precision == CMG_PREC_MIXED) ? 2 : 6;
+    if (!std::isfinite(c->opt.rayleigh_alpha) || !std::isfinite(c->opt.rayleigh_beta))
+        return fail("cmg_create: Rayleigh damping coefficients must be finite");
     c->device = c->opt.device;
     CMG_HIP_CHECK(hipSetDevice(c->device));
     CMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -201,6 +203,7 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     c->nps = c->S > 0 ? mesh->nodes_per_surface : 0;
     c->n_groups = std::max(1, mesh->n_groups);
     c->fp32 = c->opt.precision == CMG_PREC_MIXED;
+    c->damping = c->opt.rayleigh_alpha != 0.0 || c->opt.rayleigh_beta != 0.0;
     const int64_t N = c->N, E = c->E;
     // -- validate connectivity
     for (int64_t i = 0; i < E * npe; i++)
@@ -405,10 +408,16 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     for (DevBuf* b : {&c->d, &c->vel, &c->acc, &c->guess, &c->dpred, &c->vpred, &c->inc1, &c->inc2})
         launch_fill(c->n, 0.0, b->as<double>(), s);
     CMG_TRY(alloc(c->fe, (size_t)E * 3 * npe * sizeof(double)));
-    CMG_TRY(alloc(c->Ke, (size_t)E * 9 * npe * npe * (c->fp32 ? sizeof(float) : sizeof(double))));
+    // FP64-sized in mixed precision when the Rayleigh damping matrix needs FP64 element tangents
+    const bool ke64 = !c->fp32 || (c->damping && c->opt.rayleigh_beta != 0.0);
+    CMG_TRY(alloc(c->Ke, (size_t)E * 9 * npe * npe * (ke64 ? sizeof(double) : sizeof(float))));
     CMG_TRY(alloc(c->vals, (size_t)(nnzb + kSpmvPadBlocks) * 9 * (c->fp32 ? sizeof(float) : sizeof(double))));
     launch_fill_int((nnzb + kSpmvPadBlocks) * 9 * (c->fp32 ? 1 : 2), 0, c->vals.as<int>(), s);   // defined padding
     CMG_TRY(alloc(c->dinv, (size_t)N * 9 * sizeof(double)));
+    if (c->damping) {
+        CMG_TRY(alloc(c->cvals, (size_t)(nnzb + kSpmvPadBlocks) * 9 * sizeof(double)));
+        launch_fill((nnzb + kSpmvPadBlocks) * 9, 0.0, c->cvals.as<double>(), s);
+    }
     CMG_TRY(alloc(c->corrupt, sizeof(int)));
     c->gm_m = std::max(8, std::min(c->opt.krylov_restart, 1000));
     CMG_TRY(alloc(c->gm_V, (size_t)(c->gm_m + 2) * vb));
@@ -499,11 +508,14 @@ SurfArgs surf_args(cmg_ctx* c, const double* x_dev, bool with_volume) {
 namespace {
 
 // element kernels (+ surfaces) at x_dev; returns CMG_SUCCESS or CMG_CORRUPT_ELEMENT
-int eval_elements(cmg_ctx* c, const double* x_dev, double time, bool tangent, bool pressure, int* corrupt_id) {
+int eval_elements(cmg_ctx* c, const double* x_dev, double time, bool tangent, bool pressure, int* corrupt_id,
+                  bool fp64_tangent = false) {
     hipStream_t s = c->stream;
     launch_fill_int(1, INT_MAX, c->corrupt.as<int>(), s);
     ElemArgs a = elem_args(c, x_dev, time);
-    if (tangent)
+    if (tangent && fp64_tangent)
+        launch_element_residual_tangent(c->npe, false, a, s, 6);
+    else if (tangent)
         launch_element_residual_tangent(c->npe, c->fp32, a, s, c->opt.element_variant);
     else
         launch_element_residual(c->npe, a, s);
@@ -1191,7 +1203,10 @@ int cmg_set_state(cmg_ctx* c, const double* nodes, const double* vel, const doub
     if (!c) return fail("null context");
     c->trial_pending = false;
     const size_t vb = c->n * sizeof(double);
-    if (nodes) CMG_HIP_CHECK(hipMemcpyAsync(c->nodes.p, nodes, vb, hipMemcpyHostToDevice, c->stream));
+    if (nodes) {
+        CMG_HIP_CHECK(hipMemcpyAsync(c->nodes.p, nodes, vb, hipMemcpyHostToDevice, c->stream));
+        c->damping_valid = false;
+    }
     if (vel) CMG_HIP_CHECK(hipMemcpyAsync(c->vel.p, vel, vb, hipMemcpyHostToDevice, c->stream));
     if (acc) CMG_HIP_CHECK(hipMemcpyAsync(c->acc.p, acc, vb, hipMemcpyHostToDevice, c->stream));
     CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1229,9 +1244,47 @@ constexpr double kMinDomainLambda = 1.0 / (1 << 20);
 // step as diverged (SNES_DIVERGED_DTOL)
 constexpr double kNewtonDivTol = 1e4;
 
+// Rayleigh damping matrix C = alpha M + beta_R K(x_n) for the configuration in nodes
+// (CBSolverNewmarkBeta::CalcDampingMatrix, :657-702, called at the start of each new
+// time step, :857-866): K is the element tangent without surface terms, evaluated at
+// the step's time, in FP64 also in mixed precision (C w cancels strongly for the
+// near-incompressible wall, so FP32 entries would cost ~1e-4 in the damping force);
+// rebuilt only after nodes changed
+int build_damping(cmg_ctx* c, double time) {
+    if (!c->damping || c->damping_valid) return CMG_SUCCESS;
+    hipStream_t s = c->stream;
+    if (c->opt.rayleigh_beta != 0.0) {
+        int corrupt = -1;
+        const int rc = eval_elements(c, c->nodes.as<double>(), time, true, false, &corrupt, true);
+        if (rc != CMG_SUCCESS) return rc;
+        if (c->S > 0) launch_fill((int64_t)c->S * 9 * c->nps * c->nps, 0.0, c->Ks.as<double>(), s);
+        launch_gather_matrix(c->nnzb, c->cptr.as<int64_t>(), c->contrib.as<int64_t>(), c->Ke.p, false,
+                             c->Ks.as<double>(), c->n_solid_blocks, c->blkrow.as<int32_t>(), c->colind.as<int32_t>(),
+                             nullptr, 0.0, c->bcmask.as<uint8_t>(), c->cvals.p, false, s);
+    }
+    launch_rayleigh_finish(c->nnzb, c->blkrow.as<int32_t>(), c->colind.as<int32_t>(), c->bcmask.as<uint8_t>(),
+                           c->mval.as<double>(), c->opt.rayleigh_beta, c->opt.rayleigh_alpha, c->cvals.as<double>(), s);
+    CMG_HIP_CHECK(hipGetLastError());
+    c->damping_valid = true;
+    return CMG_SUCCESS;
+}
+
+// r += C (v~ + cd (d - d~)) on free DOFs, cd = gamma / (beta dt) (:384-390)
+void add_damping_force(cmg_ctx* c, double cd, double* r) {
+    if (cd == 0.0 || !c->damping) return;
+    hipStream_t s = c->stream;
+    double* w = c->tmpB.as<double>();
+    double* cw = c->tmpA.as<double>();
+    launch_waxpby(c->n, 1.0, c->d.as<double>(), -1.0, c->dpred.as<double>(), w, s);
+    launch_axpby(c->n, 1.0, c->vpred.as<double>(), cd, w, s);
+    launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), c->cvals.p, false, w, cw, s, 16, kSpmvBlock);
+    launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), cw, s);
+    launch_axpby(c->n, 1.0, cw, 1.0, r, s);
+}
+
 // ||r(d)|| of the Newmark residual at the current increment d (infinite when an
 // element inverts)
-int residual_norm_at(cmg_ctx* c, double time, double cm, double* out) {
+int residual_norm_at(cmg_ctx* c, double time, double cm, double cd, double* out) {
     hipStream_t s = c->stream;
     launch_waxpby(c->n, 1.0, c->nodes.as<double>(), 1.0, c->d.as<double>(), c->xeval.as<double>(), s);
     int corrupt = -1;
@@ -1243,6 +1296,7 @@ int residual_norm_at(cmg_ctx* c, double time, double cm, double* out) {
     if (cm != 0.0)
         launch_add_mass_term(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), c->mval.as<double>(), cm,
                              c->d.as<double>(), c->dpred.as<double>(), c->bcmask.as<uint8_t>(), c->r.as<double>(), s);
+    add_damping_force(c, cd, c->r.as<double>());
     return norm2(c, c->r.as<double>(), out);
 }
 
@@ -1251,7 +1305,7 @@ int residual_norm_at(cmg_ctx* c, double time, double cm, double* out) {
 // convergence on abstol / rtol / stol = Solver.Precision, a fresh exact tangent
 // per iteration, an inexact Krylov solve (Eisenstat-Walker forcing) and
 // backtracking on the residual norm.
-int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
+int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     double* d = c->d.as<double>();
@@ -1300,6 +1354,7 @@ int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
         if (cm != 0.0)
             launch_add_mass_term(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), c->mval.as<double>(), cm, d,
                                  c->dpred.as<double>(), c->bcmask.as<uint8_t>(), r, s);
+        add_damping_force(c, cd, r);
         CMG_TRY(norm2(c, r, &rnorm));
         t_tan += now_s() - ta;
         if (!std::isfinite(rnorm)) {
@@ -1349,6 +1404,7 @@ int newton(cmg_ctx* c, double time, double cm, cmg_step_stats* st) {
         rprev = rnorm;
         const double tb = now_s();
         gather_tangent(c, true, cm, c->vals.p);
+        if (cd != 0.0 && c->damping) launch_add_scaled_values(c->nnzb, cd, c->cvals.as<double>(), c->vals.p, c->fp32, s);
         t_tan += now_s() - tb;
         const double tk = now_s();
         launch_scale_copy(n, -1.0, r, c->dsave.as<double>(), s);
@@ -1413,13 +1469,14 @@ int cmg_static_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
         st->guess_source = 1;
     }
     launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
-    int rc = newton(c, time, 0.0, st);
+    int rc = newton(c, time, 0.0, 0.0, st);
     if (rc != CMG_SUCCESS) {
         launch_fill(n, 0.0, d, s);   // :303-306
         CMG_HIP_CHECK(hipStreamSynchronize(s));
         return CMG_FAILED;
     }
     launch_axpby(n, 1.0, d, 1.0, c->nodes.as<double>(), s);   // :359
+    c->damping_valid = false;
     CMG_HIP_CHECK(hipStreamSynchronize(s));
     return CMG_SUCCESS;
 }
@@ -1443,6 +1500,14 @@ int cmg_newmark_solve(cmg_ctx* c, double time, double dt, int32_t guess, cmg_ste
     launch_newmark_predict(n, dt, beta, gamma, c->vel.as<double>(), c->acc.as<double>(), c->dpred.as<double>(),
                            c->vpred.as<double>(), s);
     const double cm = 1.0 / (beta * dt * dt);
+    const double cd = gamma / (beta * dt);
+    {
+        const int rc = build_damping(c, time);
+        if (rc != CMG_SUCCESS) {
+            st->converged_reason = -6;
+            return CMG_FAILED;
+        }
+    }
     double* d = c->d.as<double>();
     if (from_trial) {
         st->guess_source = 5;   // d still holds the last trial increment
@@ -1451,10 +1516,10 @@ int cmg_newmark_solve(cmg_ctx* c, double time, double dt, int32_t guess, cmg_ste
         double r_pred, r_prop;
         launch_scale_copy(n, 1.0, c->dpred.as<double>(), d, s);
         launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
-        CMG_TRY(residual_norm_at(c, time, cm, &r_pred));
+        CMG_TRY(residual_norm_at(c, time, cm, cd, &r_pred));
         launch_scale_copy(n, 1.0, c->override_guess.as<double>(), d, s);
         launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
-        CMG_TRY(residual_norm_at(c, time, cm, &r_prop));
+        CMG_TRY(residual_norm_at(c, time, cm, cd, &r_prop));
         st->guess_source = (r_prop < r_pred) ? 2 : 3;
         if (st->guess_source == 3) launch_scale_copy(n, 1.0, c->dpred.as<double>(), d, s);
         st->guess_residual_ratio = r_pred > 0.0 ? r_prop / r_pred : 0.0;
@@ -1469,7 +1534,7 @@ int cmg_newmark_solve(cmg_ctx* c, double time, double dt, int32_t guess, cmg_ste
         c->override_checked = false;
     }
     launch_zero_bc(c->N, c->bcmask.as<uint8_t>(), d, s);
-    int rc = newton(c, time, cm, st);
+    int rc = newton(c, time, cm, cd, st);
     CMG_HIP_CHECK(hipStreamSynchronize(s));
     if (rc != CMG_SUCCESS) return CMG_FAILED;
     c->trial_pending = true;
@@ -1498,6 +1563,7 @@ int cmg_commit_step(cmg_ctx* c) {
     launch_newmark_correct(n, dt, beta, gamma, c->d.as<double>(), c->dpred.as<double>(), c->vpred.as<double>(),
                            c->vel.as<double>(), c->acc.as<double>(), s);
     launch_axpby(n, 1.0, c->d.as<double>(), 1.0, c->nodes.as<double>(), s);
+    c->damping_valid = false;
     // increment history (features of a learned warm start)
     launch_scale_copy(n, 1.0, c->inc1.as<double>(), c->inc2.as<double>(), s);
     launch_scale_copy(n, 1.0, c->d.as<double>(), c->inc1.as<double>(), s);

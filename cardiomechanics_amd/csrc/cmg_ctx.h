@@ -95,6 +95,10 @@ struct cmg_ctx {
     // state and work vectors
     cmg::DevBuf Xref, nodes, xeval, d, dpred, vpred, vel, acc, r, delta, dsave, tmpA, tmpB;
     cmg::DevBuf vals, dinv;
+    // Rayleigh damping matrix C (FP64 BSR values, same pattern as vals), valid for the
+    // configuration in nodes until nodes change
+    cmg::DevBuf cvals;
+    bool damping = false, damping_valid = false;
     cmg::DevBuf gm_V, gm_partial, gm_h, gm_y;
     int gm_m = 0;
     double* h_pinned = nullptr;   // host staging for reductions

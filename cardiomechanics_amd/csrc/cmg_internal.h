@@ -100,6 +100,12 @@ void launch_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* cont
                           const double* mval, double cm, const uint8_t* bcmask, void* vals, bool vals_fp32,
                           hipStream_t s, const SetStrides& b = SetStrides());   // strides: Ke, Ks, vals
 
+// Rayleigh damping C = beta_R K + alpha M in place over K's BSR values (constrained rows
+// and columns zeroed); vals += a C
+void launch_rayleigh_finish(int64_t nnzb, const int32_t* blkrow, const int32_t* colind, const uint8_t* bcmask,
+                            const double* mval, double beta_r, double alpha, double* vals, hipStream_t s);
+void launch_add_scaled_values(int64_t nnzb, double a, const double* C, void* vals, bool vals_fp32, hipStream_t s);
+
 // ---- launchers (linalg_kernels.hip) ----
 // SpMV layouts: kSpmvBlock one 3x3 block per lane step (4, 8 or 16 lanes per block
 // row); kSpmvChunk 144-byte value chunks (needs 16-byte aligned values and columns

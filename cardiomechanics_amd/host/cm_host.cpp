@@ -592,6 +592,14 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
     o.newmark_gamma = p.Get<double>("Solver.NewmarkBeta.Gamma", 0.5);
     o.consistent_mass = p.Get<bool>("Solver.NewmarkBeta.ConsistentMassMatrix", true) ? 1 : 0;
     o.fd_epsilon = p.Get<double>("Solver.Epsilon", 1e-12);
+    {
+        // CBSolverNewmarkBeta::InitDampingParameter (:274-288): the type is read under
+        // "Material.Global", the coefficients under "Materials.Global"
+        const std::string dtype = p.Get<std::string>("Material.Global.Damping.Type", "Rayleigh");
+        if (dtype != "Rayleigh") throw std::runtime_error("Global damping type " + dtype + " is unknown");
+        o.rayleigh_alpha = p.Get<double>("Materials.Global.Damping.Rayleigh.Alpha", 0.0);
+        o.rayleigh_beta = p.Get<double>("Materials.Global.Damping.Rayleigh.Beta", 0.0);
+    }
     o.precision = p.Get<std::string>("Solver.GPU.Precision", "FP64") == "Mixed" ? CMG_PREC_MIXED : CMG_PREC_FP64;
     o.krylov_rtol = p.Get<double>("Solver.GPU.KrylovTolerance", o.krylov_rtol);
     o.krylov_restart = p.Get<int>("Solver.GPU.KrylovRestart", o.krylov_restart);

@@ -228,7 +228,10 @@ def run(ctx, kind: str, timing: Timing, plugin=None, on_step=None, warm_start=No
                 time = t
                 break
             if not ok:
-                raise RuntimeError(f"minimum time step reached at t={t}: simulation failed")
+                raise RuntimeError(f"minimum time step reached at t={t}: simulation failed (last attempt: "
+                                   f"dt={timing.dt:.3e}, Newton reason {st.converged_reason} after "
+                                   f"{st.newton_iterations} iterations, ||r0||={st.initial_residual_norm:.3e}, "
+                                   f"||r||={st.residual_norm:.3e}, corrupt element {st.corrupt_element})")
         if first:
             first = False
             timing.reset_for_sim()

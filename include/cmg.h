@@ -130,7 +130,14 @@ typedef struct cmg_options {
     double newmark_beta;        /* Solver.NewmarkBeta.Beta  (default 0.25) */
     double newmark_gamma;       /* Solver.NewmarkBeta.Gamma (default 0.5) */
     double fd_epsilon;          /* Solver.Epsilon (default 1e-12), reported only */
-    double reserved[4];
+    double rayleigh_alpha;      /* Materials.Global.Damping.Rayleigh.Alpha (1/s, default 0) */
+    double rayleigh_beta;       /* Materials.Global.Damping.Rayleigh.Beta (s, default 0): Newmark steps
+                                   add C (v~ + gamma/(beta dt) (d - d~)) to the residual and
+                                   gamma/(beta dt) C to the tangent, C = alpha M + beta_R K(x_n) with
+                                   K the element tangent at the step's start configuration, rebuilt
+                                   whenever that configuration changes (CBSolverNewmarkBeta.cpp:
+                                   384-390, 622-626, 657-702, 857-866); off when both are 0 */
+    double reserved[2];
 } cmg_options;
 
 typedef struct cmg_solve_stats {

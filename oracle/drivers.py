@@ -10,7 +10,11 @@ for the reference's MUMPS LU, CBSolverNewmarkBeta.cpp:176-192):
                           identity on constrained DOFs (CBSolver.cpp:413-435)
   static_step             CBSolverEquilibrium::SolverStep / UpdateInitialGuess (:234-363)
   newmark_step            CBSolverNewmarkBeta::SolverStep (CBSolverNewmarkBeta.cpp:829-1019):
-                          predictor, Newton on M(d - d~)/(beta dt^2) + f, corrector
+                          predictor, Newton on M(d - d~)/(beta dt^2) + f, corrector;
+                          with Rayleigh damping C = alpha M + beta_R K(x_n) (:657-702,
+                          rebuilt at each new step from the step's start configuration,
+                          :857-866) the residual adds C (v~ + gamma/(beta dt)(d - d~))
+                          (:384-390) and the tangent gamma/(beta dt) C (:622-626)
 Newton iterates to a residual far below the GPU path's tolerance so that both
 converge to the same root.
 """
@@ -24,7 +28,7 @@ import oracle as O
 
 
 class CpuMechanics:
-    def __init__(self, mesh, materials, pressure_tangent=True, consistent_mass=True):
+    def __init__(self, mesh, materials, pressure_tangent=True, consistent_mass=True, rayleigh=(0.0, 0.0)):
         self.mesh = mesh
         self.materials = list(materials)
         self.ed = O.ElementData(mesh)
@@ -42,6 +46,7 @@ class CpuMechanics:
             self.sgroup = np.searchsorted(self.group_ids, mesh.surface_id)
         self.pressures = np.zeros(max(1, len(getattr(self, "group_ids", [0]))))
         self.M = self._mass(consistent_mass)
+        self.rayleigh = (float(rayleigh[0]), float(rayleigh[1]))
         self.nodes = mesh.coords.ravel().copy()
         self.vel = np.zeros(self.n)
         self.acc = np.zeros(self.n)
@@ -94,7 +99,7 @@ class CpuMechanics:
             raise FloatingPointError("corrupt element")
         return Ke
 
-    def tangent(self, x, time, include_pressure=True, cm=0.0):
+    def tangent(self, x, time, include_pressure=True, cm=0.0, extra=None):
         Ke = self.element_tangents(x, time)
         nd = Ke.shape[1]
         rows = np.repeat(self.dofs, nd, axis=1).ravel()
@@ -112,6 +117,8 @@ class CpuMechanics:
         K = sp.csr_matrix((vals, (rows, cols)), shape=(self.n, self.n))
         if cm != 0.0:
             K = K + cm * self.M
+        if extra is not None:
+            K = K + extra
         return self.apply_bc(K)
 
     def apply_bc(self, K):
@@ -122,24 +129,47 @@ class CpuMechanics:
         return K.tocsr()
 
     # -- Newton -----------------------------------------------------------------
-    def newton(self, time, d, cm=0.0, dpred=None, tol=1e-12, atol=1e-12, maxit=40):
+    def damping_matrix(self, time):
+        """C = alpha M + beta_R K(x_n): K the element tangent without surface terms at
+        the current nodes and the given time; constrained rows and columns zero."""
+        alpha, beta_r = self.rayleigh
+        C = alpha * self.M
+        if beta_r != 0.0:
+            Ke = self.element_tangents(self.nodes, time)
+            nd = Ke.shape[1]
+            K = sp.csr_matrix((Ke.ravel(), (np.repeat(self.dofs, nd, axis=1).ravel(),
+                                            np.tile(self.dofs, (1, nd)).ravel())), shape=(self.n, self.n))
+            C = C + beta_r * K
+        D = sp.diags((~self.bcdof).astype(float))
+        return (D @ C @ D).tocsr()
+
+    def _dynamic_terms(self, d, cm, dpred, damp):
+        """inertia cm M (d - d~) and damping C (v~ + cd (d - d~)) on free DOFs"""
+        t = cm * (self.M @ (d - dpred))
+        if damp is not None:
+            C, cd, vpred = damp
+            t = t + C @ (vpred + cd * (d - dpred))
+        t[self.bcdof] = 0.0
+        return t
+
+    def newton(self, time, d, cm=0.0, dpred=None, tol=1e-12, atol=1e-12, maxit=40, damp=None):
         """Full Newton with a direct solve until ||r|| <= max(tol ||r0||, atol) or the
-        Newton step falls below 1e-14 ||d|| (the residual's rounding floor)."""
+        Newton step falls below 1e-14 ||d|| (the residual's rounding floor).  damp =
+        (C, gamma/(beta dt), v~) adds Rayleigh damping."""
         r0 = None
         step = np.inf
+        extra = damp[1] * damp[0] if damp is not None else None
         for it in range(maxit + 1):
             x = self.nodes + d
             r = self.residual(x, time)
             if cm != 0.0:
-                inert = cm * (self.M @ (d - dpred))
-                inert[self.bcdof] = 0.0
-                r = r + inert
+                r = r + self._dynamic_terms(d, cm, dpred, damp)
             rn = np.linalg.norm(r)
             if r0 is None:
                 r0 = rn
             if rn <= max(tol * r0, atol) or rn == 0.0 or step <= 1e-14 * np.linalg.norm(d):
                 return d, it, rn
-            K = self.tangent(x, time, cm=cm)
+            K = self.tangent(x, time, cm=cm, extra=extra)
             delta = spla.spsolve(K.tocsc(), -r)
             lam = 1.0
             while True:
@@ -147,9 +177,7 @@ class CpuMechanics:
                 try:
                     rr = self.residual(self.nodes + dn, time)
                     if cm != 0.0:
-                        ii = cm * (self.M @ (dn - dpred))
-                        ii[self.bcdof] = 0.0
-                        rr = rr + ii
+                        rr = rr + self._dynamic_terms(dn, cm, dpred, damp)
                     if np.linalg.norm(rr) <= rn or lam < 1.0 / 64:
                         break
                 except FloatingPointError:
@@ -185,7 +213,10 @@ class CpuMechanics:
         pend = getattr(self, "_pending", None)
         d0 = pend["d"].copy() if (from_trial and pend is not None and pend["dt"] == dt) else dpred.copy()
         self._pending = None
-        d, it, rn = self.newton(time, d0, cm=cm, dpred=dpred, tol=tol)
+        damp = None
+        if self.rayleigh != (0.0, 0.0):
+            damp = (self.damping_matrix(time), gamma / (beta * dt), vpred)
+        d, it, rn = self.newton(time, d0, cm=cm, dpred=dpred, tol=tol, damp=damp)
         self._pending = dict(d=d, dpred=dpred, vpred=vpred, dt=dt, beta=beta, gamma=gamma)
         return it, rn
 

@@ -332,3 +332,40 @@ def test_full_size_lv_properties():
     rc, fo, _ = O.residual_batch(oracle_elements(sub), x, mats, 0.1)
     err = np.abs(fe[sample] - fo) / np.abs(fo).max(axis=1, keepdims=True)
     assert err.max() <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_newmark_steps_with_rayleigh_damping_match_oracle(precision):
+    """Rayleigh damping (the reference LV settings' alpha = 500 1/s, beta_R = 0.01 s):
+    every step against the oracle's damped Newmark step; the damping is active (the
+    undamped oracle run is far away)."""
+    mesh, mats = lv_small()
+    damp = (500.0, 0.01)
+    cpu = CpuMechanics(mesh, mats, rayleigh=damp)
+    free = CpuMechanics(mesh, mats)
+    plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
+    dt = 0.005
+    # mixed precision builds the damping matrix from FP64 element tangents and keeps the
+    # residual in FP64: it converges to the same root
+    tol = 1e-8
+    with MechanicsContext(mesh, mats, newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15,
+                          precision=precision, rayleigh_alpha=damp[0], rayleigh_beta=damp[1]) as ctx:
+        for k in range(1, 9):
+            t = k * dt
+            p = plug.apply(t, dt)[0]
+            ctx.set_pressures([p])
+            cpu.pressures[:] = p
+            free.pressures[:] = p
+            rc, st = ctx.newmark_step(t, dt)
+            assert rc == 2, st.as_dict()
+            cpu.newmark_step(t, dt)
+            free.newmark_step(t, dt)
+            s = ctx.get_state()
+            u = cpu.nodes - mesh.coords.ravel()
+            err = np.linalg.norm(s["nodes"] - cpu.nodes) / np.linalg.norm(u)
+            verr = np.linalg.norm(s["velocity"] - cpu.vel) / np.linalg.norm(cpu.vel)
+            gap = np.linalg.norm(free.nodes - cpu.nodes) / np.linalg.norm(u)
+            print(f"t={t:.3f} newton={st.newton_iterations} krylov={st.krylov_iterations} disp err={err:.2e} "
+                  f"vel err={verr:.2e} damped vs undamped={gap:.2e}")
+            assert err <= tol and verr <= 100 * tol
+            assert gap > 1e3 * tol

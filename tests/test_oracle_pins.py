@@ -451,3 +451,53 @@ def test_lv_generator_orients_the_endocardium_like_the_reference_mesh():
     cpu.static_step(0.0, 1.0)
     cpu.static_step(1.0, 1.0)
     assert v0 < 0 and cpu.cavity_volumes()[0] < v0      # |V| grows
+
+
+@pytest.mark.parametrize("alpha_over_w,beta_r_w", [(0.2, 0.0), (0.0, 0.2), (0.1, 0.1)])
+def test_rayleigh_damping_reduces_to_the_damped_oscillator(alpha_over_w, beta_r_w):
+    """Small vibrations of the clamped cube from rest in its lowest mode phi (K0 phi =
+    w^2 M phi): C = alpha M + beta_R K is diagonal in the modes, so the modal coordinate
+    q = phi^T M u of the oracle's Newmark run equals Newmark on q'' + (alpha + beta_R
+    w^2) q' + w^2 q = 0 step for step, and follows the exact damped oscillation to the
+    trapezoidal rule's O((w dt)^2) phase error."""
+    import scipy.linalg as sla
+    from oracle.drivers import CpuMechanics
+
+    mesh, mats = M.cube_mesh(n=2), [MT.cube_guccione()]
+    cpu = CpuMechanics(mesh, mats)
+    X = mesh.coords.ravel().copy()
+    free = ~cpu.bcdof
+    K0 = cpu.tangent(X, 0.0, include_pressure=False).toarray()[np.ix_(free, free)]
+    Mf = cpu.M.toarray()[np.ix_(free, free)]
+    w2, V = sla.eigh(0.5 * (K0 + K0.T), Mf)
+    w = math.sqrt(w2[0])
+    phi = np.zeros(cpu.n)
+    phi[free] = V[:, 0]                                   # phi^T M phi = 1
+    alpha, beta_r = alpha_over_w * w, beta_r_w / w
+    cpu = CpuMechanics(mesh, mats, rayleigh=(alpha, beta_r))
+    eps = 1e-8 / np.abs(phi).max()
+    cpu.nodes = X + eps * phi
+    cpu.acc = -w2[0] * eps * phi
+    dt = (2 * math.pi / w) / 40
+    beta, gamma = 0.25, 0.5
+    c, k = alpha + beta_r * w2[0], w2[0]
+    q, v, a = eps, 0.0, -k * eps
+    zw = 0.5 * c
+    wd = math.sqrt(k - zw * zw)
+    for n in range(1, 41):
+        cpu.newmark_step(n * dt, dt, tol=1e-13)
+        # the same Newmark step on the modal equation
+        dp = dt * v + dt * dt * 0.5 * (1 - 2 * beta) * a
+        vp = v + (1 - gamma) * dt * a
+        m1, c1 = 1 / (beta * dt * dt), gamma / (beta * dt)
+        inc = (m1 * dp + c * c1 * dp - c * vp - k * q) / (m1 + c * c1 + k)
+        a = m1 * (inc - dp)
+        v = vp + gamma * dt * a
+        q = q + inc
+        qf = phi @ (cpu.M @ (cpu.nodes - X))
+        assert abs(qf - q) <= 1e-4 * eps, (n, qf / eps, q / eps)
+        t = n * dt
+        exact = eps * math.exp(-zw * t) * (math.cos(wd * t) + zw / wd * math.sin(wd * t))
+        assert abs(q - exact) <= 2e-2 * eps, (n, q / eps, exact / eps)
+    # damping ratio 0.1: after one period the amplitude is ~0.53 of the undamped one
+    assert abs(q - eps * math.cos(w * 40 * dt)) > 0.3 * eps
