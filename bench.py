@@ -27,8 +27,8 @@ BYTES_PER_ELEM = {0: 1548 + 240 + 7200, 1: 1548 + 240 + 3600}
 FLOPS_PER_ELEM = 42_700
 # measured HBM bytes per element of the default FP64 kernel (variant 6): rocprofv3 FETCH_SIZE +
 # WRITE_SIZE per dispatch, calibrated on a copy kernel of known size
-# (profiles/element_residual_tangent_lv197k_fp64.md); variant 4 measured 9,093
-TRAFFIC_PER_ELEM = {0: 9_082, 1: None}
+# on the 200,340-element LV (profiles/element_residual_tangent_lv200k_fp64.md)
+TRAFFIC_PER_ELEM = {0: 8_881, 1: None}
 HBM_PEAK = 8.0e12
 
 
