@@ -4,8 +4,8 @@
 // phase) to that step's displacement increment.  Hand-written gfx950 kernels for
 // inference, the backward pass (activation gradients and weight-gradient
 // reductions, two-stage and in a fixed order) and Adam.  The step only uses the
-// proposed guess when its residual is below the predictor's (the predictor is the
-// fallback and the judge).
+// proposed guess when its residual is below the predictor's (the predictor is both
+// the fallback and the yardstick).
 //
 // Weight layout (FP32, flat): W1[8][H] b1[H] W2[H][H] b2[H] W3[H][3] b3[3]
 // (W[k][c]: input k -> output c).
