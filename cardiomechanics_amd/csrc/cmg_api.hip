@@ -1401,6 +1401,10 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
             break;
         }
         if (it > 0) eta = std::min(1e-1, std::max(c->opt.krylov_rtol, 0.9 * (rnorm / rprev) * (rnorm / rprev)));
+        // no oversolving: a linear residual below half the nonlinear tolerance is all the
+        // next iterate needs (Kelley 1995, eq. 6.20: eta >= 0.5 tau_t / ||r||, tau_t =
+        // atol + rtol ||r_0||)
+        const double eta_k = std::min(1e-1, std::max(eta, 0.5 * (c->opt.newton_atol + c->opt.newton_rtol * r0) / rnorm));
         rprev = rnorm;
         const double tb = now_s();
         gather_tangent(c, true, cm, c->vals.p);
@@ -1410,7 +1414,7 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
         launch_scale_copy(n, -1.0, r, c->dsave.as<double>(), s);
         launch_fill(n, 0.0, delta, s);
         cmg_solve_stats ls;
-        CMG_TRY(krylov(c, c->vals.p, c->dsave.as<double>(), delta, eta, c->opt.krylov_maxit, &ls));
+        CMG_TRY(krylov(c, c->vals.p, c->dsave.as<double>(), delta, eta_k, c->opt.krylov_maxit, &ls));
         kry_total += ls.iterations;
         t_kry += now_s() - tk;
         lambda = 1.0;
