@@ -285,6 +285,7 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
     const cmg_options& o = c->opt;
     std::vector<int32_t> run(S, 1), its(S, 0), kry(S, 0), reason(S, 0), zero_tried(S, 0), corrupt;
     std::vector<double> lam(S, 1.0), rprev(S, 0.0), r0(S, 0.0), rn(S, 0.0), sn(S, 0.0), dn(S, 0.0), eta(S, 1e-2);
+    std::vector<double> eta_k(S, 1e-2);
     const double t0 = now_s();
     double t_tan = 0.0, t_kry = 0.0;
     std::vector<double> dots;
@@ -349,9 +350,12 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
                 any_back = true;
                 continue;
             }
+            // the single-set Newton's tests (cmg_api.hip newton()): divergence tolerance,
+            // step-norm convergence only after a residual decrease
             if (rn[i] < o.newton_atol) reason[i] = 2;
+            else if (its[i] > 0 && rn[i] > 1e4 * r0[i]) reason[i] = -9;
             else if (its[i] > 0 && rn[i] < o.newton_rtol * r0[i]) reason[i] = 3;
-            else if (its[i] > 0 && sn[i] * lam[i] < o.newton_stol * dn[i]) reason[i] = 4;
+            else if (its[i] > 0 && rn[i] <= rprev[i] && sn[i] * lam[i] < o.newton_stol * dn[i]) reason[i] = 4;
             else if (its[i] >= o.newton_maxit) reason[i] = -5;
             if (reason[i] != 0) {
                 run[i] = 0;
@@ -359,6 +363,8 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
             }
             if (its[i] > 0)
                 eta[i] = std::min(1e-1, std::max(o.krylov_rtol, 0.9 * (rn[i] / rprev[i]) * (rn[i] / rprev[i])));
+            // Kelley's safeguard, as in the single-set Newton
+            eta_k[i] = std::min(1e-1, std::max(eta[i], 0.5 * (o.newton_atol + o.newton_rtol * r0[i]) / rn[i]));
             rprev[i] = rn[i];
             solve[i] = 1;
         }
@@ -371,7 +377,7 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
         t_tan += now_s() - tb;
         const double tk = now_s();
         std::vector<int32_t> cits;
-        CMG_TRY(bcg_solve(b, eta, cits));
+        CMG_TRY(bcg_solve(b, eta_k, cits));
         t_kry += now_s() - tk;
         std::vector<double> one(S, 0.0);
         for (int i = 0; i < S; i++)
@@ -386,8 +392,12 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
                       dots));
         for (int i = 0; i < S; i++)
             if (solve[i]) {
-                sn[i] = std::sqrt(std::max(dots[i], 0.0));
+                sn[i] = std::isfinite(dots[i]) ? std::sqrt(std::max(dots[i], 0.0)) : INFINITY;
                 dn[i] = std::sqrt(std::max(dots[S + i], 0.0));
+                if (!(sn[i] > 0.0) || !std::isfinite(sn[i])) {   // no usable update from the Krylov solve
+                    reason[i] = -3;
+                    run[i] = 0;
+                }
             }
     }
     const double wall = now_s() - t0;
