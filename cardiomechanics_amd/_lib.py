@@ -24,7 +24,7 @@ EXPORTED = [
     "cmg_element_residuals", "cmg_element_tangents", "cmg_surface_forces", "cmg_cavity_volumes",
     "cmg_mass_matrix", "cmg_element_export", "cmg_spmv_tuning", "cmg_residual_device", "cmg_residual_tangent_device", "cmg_element_kernel_device",
     "cmg_spmv", "cmg_spmv_layout", "cmg_linear_solve", "cmg_set_state", "cmg_get_state", "cmg_set_initial_guess",
-    "cmg_static_step", "cmg_newmark_step", "cmg_newmark_solve", "cmg_trial_volumes", "cmg_commit_step",
+    "cmg_static_step", "cmg_newmark_step", "cmg_newmark_energies", "cmg_newmark_solve", "cmg_trial_volumes", "cmg_commit_step",
     "cmg_copy_device",
     "cmg_batch_create", "cmg_batch_destroy", "cmg_batch_configure", "cmg_batch_info_get", "cmg_batch_static_step",
     "cmg_batch_get_nodes", "cmg_batch_set_nodes", "cmg_batch_residual", "cmg_batch_cavity_volumes",
@@ -137,6 +137,7 @@ def load():
         "cmg_set_initial_guess": (ctypes.c_int, [P, P]),
         "cmg_static_step": (ctypes.c_int, [P, dbl, dbl, P]),
         "cmg_newmark_step": (ctypes.c_int, [P, dbl, dbl, P]),
+        "cmg_newmark_energies": (ctypes.c_int, [P, P, P]),
         "cmg_newmark_solve": (ctypes.c_int, [P, dbl, dbl, ctypes.c_int32, P]),
         "cmg_trial_volumes": (ctypes.c_int, [P, P]),
         "cmg_commit_step": (ctypes.c_int, [P]),

@@ -263,6 +263,15 @@ class MechanicsContext:
             raise CmgError(rc, "cmg_newmark_step")
         return rc, st
 
+    def newmark_energies(self):
+        """(kinetic energy after the last committed step, accumulated Rayleigh damping
+        dissipation), as the reference's Newmark corrector records them."""
+        ke, ed = ctypes.c_double(), ctypes.c_double()
+        rc = self.L.cmg_newmark_energies(self.h, ctypes.byref(ke), ctypes.byref(ed))
+        if rc != _lib.SUCCESS:
+            raise CmgError(rc, "cmg_newmark_energies")
+        return ke.value, ed.value
+
     def newmark_solve(self, time, dt, from_trial=False):
         """Predictor + Newton of a Newmark step, kept pending (see commit_step)."""
         st = _lib.CmgStepStats()

@@ -1566,6 +1566,24 @@ int cmg_commit_step(cmg_ctx* c) {
     // corrector (:984-1015)
     launch_newmark_correct(n, dt, beta, gamma, c->d.as<double>(), c->dpred.as<double>(), c->vpred.as<double>(),
                            c->vel.as<double>(), c->acc.as<double>(), s);
+    // kinetic energy v^T M v / 2 and damping dissipation (C v) . d (:990-1003), C
+    // still that of the step's start configuration
+    {
+        double* mv = c->tmpA.as<double>();
+        launch_fill(n, 0.0, mv, s);
+        launch_fill(n, 0.0, c->tmpB.as<double>(), s);
+        launch_add_mass_term(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), c->mval.as<double>(), 1.0,
+                             c->vel.as<double>(), c->tmpB.as<double>(), c->bcmask.as<uint8_t>(), mv, s);
+        double h = 0.0;
+        CMG_TRY(multidot(c, 1, mv, 0, c->vel.as<double>(), &h));
+        c->kinetic_energy = 0.5 * h;
+        if (c->damping) {
+            launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), c->cvals.p, false,
+                        c->vel.as<double>(), mv, s, 16, kSpmvBlock);
+            CMG_TRY(multidot(c, 1, mv, 0, c->d.as<double>(), &h));
+            c->damping_energy += h;
+        }
+    }
     launch_axpby(n, 1.0, c->d.as<double>(), 1.0, c->nodes.as<double>(), s);
     c->damping_valid = false;
     // increment history (features of a learned warm start)
@@ -1573,6 +1591,13 @@ int cmg_commit_step(cmg_ctx* c) {
     launch_scale_copy(n, 1.0, c->d.as<double>(), c->inc1.as<double>(), s);
     c->trial_pending = false;
     CMG_HIP_CHECK(hipStreamSynchronize(s));
+    return CMG_SUCCESS;
+}
+
+int cmg_newmark_energies(cmg_ctx* c, double* kinetic, double* damping_dissipated) {
+    if (!c) return fail("null context");
+    if (kinetic) *kinetic = c->kinetic_energy;
+    if (damping_dissipated) *damping_dissipated = c->damping_energy;
     return CMG_SUCCESS;
 }
 

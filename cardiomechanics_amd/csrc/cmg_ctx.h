@@ -99,6 +99,8 @@ struct cmg_ctx {
     // configuration in nodes until nodes change
     cmg::DevBuf cvals;
     bool damping = false, damping_valid = false;
+    // corrector energies of the last committed Newmark step
+    double kinetic_energy = 0.0, damping_energy = 0.0;
     cmg::DevBuf gm_V, gm_partial, gm_h, gm_y;
     int gm_m = 0;
     double* h_pinned = nullptr;   // host staging for reductions

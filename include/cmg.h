@@ -270,6 +270,12 @@ int cmg_static_step(cmg_ctx* ctx, double time, double dt, cmg_step_stats* stats)
 /* CBSolverNewmarkBeta::SolverStep (CBSolverNewmarkBeta.cpp:829-1019): predictor,
  * Newton on M(d - d~)/(beta dt^2) + f_int + f_p = 0, corrector. */
 int cmg_newmark_step(cmg_ctx* ctx, double time, double dt, cmg_step_stats* stats);
+/* Energies the Newmark corrector records (CBSolverNewmarkBeta.cpp:990-1003): the
+ * kinetic energy v^T M v / 2 after the last committed step, and the Rayleigh damping
+ * dissipation, the sum over committed steps of (C v_{n+1}) . d_{n+1} (0 without
+ * damping); the reference skips that sum while preparing the simulation, this
+ * library counts every committed step. */
+int cmg_newmark_energies(cmg_ctx* ctx, double* kinetic, double* damping_dissipated);
 /* The same step in two halves, for plugins that analyse the converged state before it
  * is accepted (CBSolverNewmarkBeta.cpp:928-976; CBCirculation::AnalyzeResults,
  * CBCirculation.cpp): cmg_newmark_solve runs the predictor and Newton and keeps the

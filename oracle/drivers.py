@@ -47,6 +47,8 @@ class CpuMechanics:
         self.pressures = np.zeros(max(1, len(getattr(self, "group_ids", [0]))))
         self.M = self._mass(consistent_mass)
         self.rayleigh = (float(rayleigh[0]), float(rayleigh[1]))
+        self.kinetic_energy = 0.0
+        self.damping_energy = 0.0
         self.nodes = mesh.coords.ravel().copy()
         self.vel = np.zeros(self.n)
         self.acc = np.zeros(self.n)
@@ -217,7 +219,8 @@ class CpuMechanics:
         if self.rayleigh != (0.0, 0.0):
             damp = (self.damping_matrix(time), gamma / (beta * dt), vpred)
         d, it, rn = self.newton(time, d0, cm=cm, dpred=dpred, tol=tol, damp=damp)
-        self._pending = dict(d=d, dpred=dpred, vpred=vpred, dt=dt, beta=beta, gamma=gamma)
+        self._pending = dict(d=d, dpred=dpred, vpred=vpred, dt=dt, beta=beta, gamma=gamma,
+                             C=damp[0] if damp is not None else None)
         return it, rn
 
     def trial_volumes(self):
@@ -229,6 +232,13 @@ class CpuMechanics:
         dt, beta, gamma = p["dt"], p["beta"], p["gamma"]
         self.acc = 1.0 / (beta * dt * dt) * p["d"] + -1.0 / (beta * dt * dt) * p["dpred"]
         self.vel = p["vpred"] + gamma * dt * self.acc
+        # kinetic energy and damping dissipation (CBSolverNewmarkBeta.cpp:990-1003);
+        # the mass matrix's constrained rows are zero
+        mv = self.M @ self.vel
+        mv[self.bcdof] = 0.0
+        self.kinetic_energy = 0.5 * float(mv @ self.vel)
+        if p["C"] is not None:
+            self.damping_energy += float((p["C"] @ self.vel) @ p["d"])
         self.d = p["d"]
         self.nodes = self.nodes + p["d"]
         self._pending = None

@@ -365,7 +365,10 @@ def test_newmark_steps_with_rayleigh_damping_match_oracle(precision):
             err = np.linalg.norm(s["nodes"] - cpu.nodes) / np.linalg.norm(u)
             verr = np.linalg.norm(s["velocity"] - cpu.vel) / np.linalg.norm(cpu.vel)
             gap = np.linalg.norm(free.nodes - cpu.nodes) / np.linalg.norm(u)
+            ke, ed = ctx.newmark_energies()
             print(f"t={t:.3f} newton={st.newton_iterations} krylov={st.krylov_iterations} disp err={err:.2e} "
-                  f"vel err={verr:.2e} damped vs undamped={gap:.2e}")
+                  f"vel err={verr:.2e} damped vs undamped={gap:.2e} kinetic={ke:.3e} J dissipated={ed:.3e} J")
             assert err <= tol and verr <= 100 * tol
             assert gap > 1e3 * tol
+            assert abs(ke - cpu.kinetic_energy) <= 100 * tol * cpu.kinetic_energy
+            assert abs(ed - cpu.damping_energy) <= 100 * tol * abs(cpu.damping_energy)

@@ -484,6 +484,8 @@ def test_rayleigh_damping_reduces_to_the_damped_oscillator(alpha_over_w, beta_r_
     q, v, a = eps, 0.0, -k * eps
     zw = 0.5 * c
     wd = math.sqrt(k - zw * zw)
+    e_scale = 0.5 * k * eps * eps          # the mode's initial potential energy
+    dissipated = 0.0
     for n in range(1, 41):
         cpu.newmark_step(n * dt, dt, tol=1e-13)
         # the same Newmark step on the modal equation
@@ -496,6 +498,11 @@ def test_rayleigh_damping_reduces_to_the_damped_oscillator(alpha_over_w, beta_r_
         q = q + inc
         qf = phi @ (cpu.M @ (cpu.nodes - X))
         assert abs(qf - q) <= 1e-4 * eps, (n, qf / eps, q / eps)
+        # corrector energies in the modal limit: v^T M v / 2 = q'^2 / 2 and
+        # (C v) . d = c q' dq (phi^T C phi = alpha + beta_R w^2)
+        dissipated += c * v * inc
+        assert abs(cpu.kinetic_energy - 0.5 * v * v) <= 1e-4 * e_scale, n
+        assert abs(cpu.damping_energy - dissipated) <= 1e-4 * e_scale, n
         t = n * dt
         exact = eps * math.exp(-zw * t) * (math.cos(wd * t) + zw / wd * math.sin(wd * t))
         assert abs(q - exact) <= 2e-2 * eps, (n, q / eps, exact / eps)
