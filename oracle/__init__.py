@@ -367,7 +367,7 @@ class ElementData:
             self.bases = np.ascontiguousarray(mesh.bases.reshape(E, 9))
         self.npe = npe
         self.conn = np.ascontiguousarray(mesh.elements, dtype=np.int32)
-        self.bcmask = np.ascontiguousarray(mesh.bc, dtype=np.uint8)
+        self.bcmask = np.array(mesh.bc, dtype=np.uint8)          # a copy: callers may edit it
         self.matidx = np.ascontiguousarray(mesh.material, dtype=np.int32)
         self.tact = np.ascontiguousarray(mesh.activation, dtype=np.float64)
 

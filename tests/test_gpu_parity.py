@@ -372,3 +372,26 @@ def test_newmark_steps_with_rayleigh_damping_match_oracle(precision):
             assert gap > 1e3 * tol
             assert abs(ke - cpu.kinetic_energy) <= 100 * tol * cpu.kinetic_energy
             assert abs(ed - cpu.damping_energy) <= 100 * tol * abs(cpu.damping_energy)
+
+
+def test_newton_reports_a_krylov_solve_without_update_as_failure():
+    """A Krylov solve that returns no update (here: krylov_maxit = 0) ends the step as
+    SNES_DIVERGED_LINEAR_SOLVE (-3); it must not read as a short step (step-norm
+    convergence) with the residual still large."""
+    mesh, mats = lv_small()
+    plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
+    dt = 0.005
+    with MechanicsContext(mesh, mats) as ctx:
+        for k in (1, 2):
+            ctx.set_pressures([plug.apply(k * dt, dt)[0]])
+            rc, st = ctx.newmark_step(k * dt, dt)
+            assert rc == 2, st.as_dict()
+        s = ctx.get_state()
+    with MechanicsContext(mesh, mats, krylov_maxit=0) as ctx:
+        ctx.set_state(nodes=s["nodes"], velocity=s["velocity"], acceleration=s["acceleration"])
+        ctx.set_pressures([plug.apply(3 * dt, dt)[0]])
+        rc, st = ctx.newmark_step(3 * dt, dt)       # non-zero predictor: d != 0
+        print(st.as_dict())
+        assert rc == -3
+        assert st.converged_reason == -3
+        assert st.initial_residual_norm > 1e-8

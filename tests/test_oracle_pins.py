@@ -508,3 +508,13 @@ def test_rayleigh_damping_reduces_to_the_damped_oscillator(alpha_over_w, beta_r_
         assert abs(q - exact) <= 2e-2 * eps, (n, q / eps, exact / eps)
     # damping ratio 0.1: after one period the amplitude is ~0.53 of the undamped one
     assert abs(q - eps * math.cos(w * 40 * dt)) > 0.3 * eps
+
+
+def test_element_data_does_not_alias_the_mesh_constraints():
+    """Editing the oracle's per-element constraint mask (element checks without
+    constraints) must leave the mesh's boundary conditions intact."""
+    mesh = M.cube_mesh(2)
+    bc = mesh.bc.copy()
+    ed = O.ElementData(mesh)
+    ed.bcmask[:] = 0
+    assert np.array_equal(mesh.bc, bc)
