@@ -284,8 +284,8 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
     const int64_t n = b->n;
     const cmg_options& o = c->opt;
     std::vector<int32_t> run(S, 1), its(S, 0), kry(S, 0), reason(S, 0), zero_tried(S, 0), corrupt;
-    std::vector<double> lam(S, 1.0), rprev(S, 0.0), r0(S, 0.0), rn(S, 0.0), sn(S, 0.0), dn(S, 0.0), eta(S, 1e-2);
-    std::vector<double> eta_k(S, 1e-2);
+    std::vector<double> lam(S, 1.0), rprev(S, 0.0), r0(S, 0.0), rn(S, 0.0), sn(S, 0.0), dn(S, 0.0), eta(S, 1e-1);
+    std::vector<double> eta_k(S, 1e-1);
     const double t0 = now_s();
     double t_tan = 0.0, t_kry = 0.0;
     std::vector<double> dots;
@@ -362,9 +362,9 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
                 continue;
             }
             if (its[i] > 0)
-                eta[i] = std::min(1e-1, std::max(o.krylov_rtol, 0.9 * (rn[i] / rprev[i]) * (rn[i] / rprev[i])));
+                eta[i] = std::min(0.5, std::max(o.krylov_rtol, 0.9 * (rn[i] / rprev[i]) * (rn[i] / rprev[i])));
             // Kelley's safeguard, as in the single-set Newton
-            eta_k[i] = std::min(1e-1, std::max(eta[i], 0.5 * (o.newton_atol + o.newton_rtol * r0[i]) / rn[i]));
+            eta_k[i] = std::min(0.5, std::max(eta[i], 0.5 * (o.newton_atol + o.newton_rtol * r0[i]) / rn[i]));
             rprev[i] = rn[i];
             solve[i] = 1;
         }
