@@ -140,7 +140,7 @@ void cmg_default_options(cmg_options* o) {
     o->krylov_maxit = 4000;
     o->krylov_method = 2;   // BiCGStab, host-side scalars: the fastest on the 200k-element LV beat
     o->device = 0;
-    o->element_variant = 0;   // by precision: 6 (FP64), 2 (mixed), the fastest of each on the LV
+    o->element_variant = 0;   // by precision: 6 (FP64), 5 (mixed), the fastest of each on the LV
     o->newton_atol = 1e-8;
     o->newton_rtol = 1e-8;
     o->newton_stol = 1e-8;
@@ -186,7 +186,7 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     else
         cmg_default_options(&c->opt);
     if (c->opt.element_variant < 1 || c->opt.element_variant > 6)
-        c->opt.element_variant = (c->opt.precision == CMG_PREC_MIXED) ? 2 : 6;
+        c->opt.element_variant = (c->opt.precision == CMG_PREC_MIXED) ? 5 : 6;
     if (!std::isfinite(c->opt.rayleigh_alpha) || !std::isfinite(c->opt.rayleigh_beta))
         return fail("cmg_create: Rayleigh damping coefficients must be finite");
     c->device = c->opt.device;

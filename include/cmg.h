@@ -118,7 +118,7 @@ typedef struct cmg_options {
     int32_t element_variant;    /* residual+tangent kernel: 2 shared quadrature stage, 3 the same in
                                    two node passes, 4 the same with two lanes per element and
                                    spatial pair, 1 per-lane quadrature points, 5 / 6 variants 2 / 4
-                                   with 14 elements per workgroup; 0 (default): 6 in FP64, 2 in
+                                   with 14 elements per workgroup; 0 (default): 6 in FP64, 5 in
                                    mixed precision */
     int32_t krylov_method;      /* 0 GMRES(restart), 1 BiCGStab with device-side scalars and graph-replayed
                                    iterations, 2 BiCGStab with host-side scalars (default); all right
