@@ -55,9 +55,11 @@ def test_cmg_driver_cube_static_matches_python_host(tmp_path):
     # their node positions agree to ~1e-12 m, not to rounding
     np.testing.assert_allclose(cpp["points"], py["points"], rtol=1e-10, atol=1e-11)
     assert set(cpp["cell_data"]) == set(py["cell_data"]) and set(cpp["point_data"]) == set(py["point_data"])
+    # fields of two solutions that agree to ~1e-12 m: strains to ~1e-12 m / 5 mm, so the
+    # penalty stress (kappa = 1e6 Pa) to ~1e-3 Pa of a ~150 Pa field
     for k in ("J", "Lambda", "f", "S_ii", "S_ij", "E_ii"):
         a, b = cpp["cell_data"][k], py["cell_data"][k]
-        np.testing.assert_allclose(a, b, rtol=1e-7, atol=1e-9 * np.abs(b).max(), err_msg=k)
+        np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-5 * np.abs(b).max(), err_msg=k)
 
 
 def test_cmg_driver_lv_windkessel_matches_python_host(tmp_path, monkeypatch):
