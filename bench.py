@@ -25,10 +25,10 @@ import numpy as np  # noqa: E402
 # f 240 + K_e 7200 (FP64) / 3600 (FP32 values)
 BYTES_PER_ELEM = {0: 1548 + 240 + 7200, 1: 1548 + 240 + 3600}
 FLOPS_PER_ELEM = 42_700
-# measured HBM bytes per element of the default FP64 kernel (variant 6): rocprofv3 FETCH_SIZE +
-# WRITE_SIZE per dispatch, calibrated on a copy kernel of known size
+# measured HBM bytes per element of the default kernels (FP64 variant 6, mixed variant 5):
+# rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, calibrated on a copy kernel of known size,
 # on the 200,340-element LV (profiles/element_residual_tangent_lv200k_fp64.md)
-TRAFFIC_PER_ELEM = {0: 8_881, 1: None}
+TRAFFIC_PER_ELEM = {0: 8_881, 1: 5_263}
 HBM_PEAK = 8.0e12
 
 
