@@ -395,3 +395,22 @@ def test_newton_reports_a_krylov_solve_without_update_as_failure():
         assert rc == -3
         assert st.converged_reason == -3
         assert st.initial_residual_norm > 1e-8
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_lv_beat_passes_activation_onset(precision):
+    """The 20k-element LV beat through activation onset (0-40 ms, Newmark 1/4, 1/2,
+    dt = 1 ms with CBTiming halving): every step is accepted with its residual below
+    Solver.Precision.  On the first LV generator's apex needles Newton failed here
+    until dt reached its minimum, and a mixed-precision run once took a broken-down
+    Krylov solve for convergence."""
+    mesh = M.lv_mesh(20_000)
+    recs = []
+    with MechanicsContext(mesh, [MT.lv_holzapfel()], precision=precision) as ctx:
+        SV.run(ctx, "newmark", SV.Timing(0.0, 0.04, 1e-3), SV.sinus_pulse(-16000.0, 0.0, 0.8),
+               on_step=lambda rec, st: recs.append((rec.time, rec.dt, st.residual_norm, st.converged_reason)))
+    halved = sum(1 for r in recs if r[1] < 0.999e-3)
+    print(f"{len(recs)} accepted steps, {halved} shorter than 1 ms")
+    assert recs[-1][0] >= 0.04 - 1e-9
+    for t, dt, rn, reason in recs:
+        assert reason > 0 and (rn < 1e-8 or reason in (3, 4)), (t, dt, rn, reason)
