@@ -63,46 +63,101 @@ __global__ __launch_bounds__(256) void k_add_mass_term(int64_t N, const int64_t*
     if (!(m & 4)) f[3 * r + 2] += cm * s2;
 }
 
+// one 3x3 block of a contribution: solid element block idx (layout [e][a][b][3][3]) or a
+// surface block (idx >= n_solid), accumulated into s[9] (transposed when trans)
+template <typename KT>
+__device__ __forceinline__ void add_block(double* s, int32_t code, const KT* Ke, const double* Ks, int32_t n_solid,
+                                          int npe, bool flip) {
+    bool trans = code < 0;
+    int32_t idx = trans ? ~code : code;
+    if (idx < n_solid) {
+        if (flip) {
+            // the same element's block (b,a) instead of (a,b), transposed: assembles K_e^T
+            const int npe2 = npe * npe;
+            const int32_t e = idx / npe2;
+            const int32_t rem = idx - e * npe2;
+            const int32_t p = rem / npe, q = rem - (rem / npe) * npe;
+            idx = e * npe2 + q * npe + p;
+            trans = !trans;
+        }
+        const KT* blk = Ke + 9 * (int64_t)idx;
+        double v[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = (double)blk[k];
+        if (trans) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) s[3 * i + j] += v[3 * j + i];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) s[k] += v[k];
+        }
+    } else {
+        // surface block: idx - n_solid = (s * nps + a) * nps + b, packed with nps in the low bits
+        const int32_t sidx = idx - n_solid;
+        const int nps = (int)(sidx & 7);
+        const int32_t q = sidx >> 3;
+        const int32_t si = q / (nps * nps);
+        const int ab = (int)(q - si * nps * nps);
+        const int a = ab / nps, b = ab - nps * (ab / nps);
+        const int n = 3 * nps;
+        const double* base = Ks + (int64_t)si * n * n + (int64_t)(3 * a) * n + 3 * b;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) s[3 * i + j] += trans ? base[(int64_t)j * n + i] : base[(int64_t)i * n + j];
+    }
+}
+
+// One thread per 3x3 block.  The first `light_blocks` workgroups take the slots in
+// pattern order and skip the heavy ones (more than kHeavyContrib contributions: the
+// diagonal blocks of element vertices, ~20-30 each); the remaining workgroups take the
+// heavy slots from their own list, so no wave idles on one long contributor list.
 template <typename KT, typename VT>
-__global__ __launch_bounds__(256) void k_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* contrib,
-                                                       const KT* Ke, const double* Ks, int64_t n_solid,
+__global__ __launch_bounds__(256) void k_gather_matrix(int64_t nnzb, const int64_t* cptr, const int32_t* contrib,
+                                                       const int32_t* heavy, int64_t n_heavy, int64_t light_blocks,
+                                                       const KT* Ke, const double* Ks, int32_t n_solid, int npe,
                                                        const int32_t* blkrow, const int32_t* colind,
                                                        const double* mval, double cm, const uint8_t* bcmask,
-                                                       VT* vals, SetStrides st) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                       VT* vals, bool flip, SetStrides st) {
     const int64_t set = blockIdx.y;
-    if (t >= 9 * nnzb || (st.active && !st.active[set])) return;
+    if (st.active && !st.active[set]) return;
+    int64_t slot;
+    if ((int64_t)blockIdx.x < light_blocks) {
+        slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (slot >= nnzb) return;
+        if (cptr[slot + 1] - cptr[slot] > kHeavyContrib) return;
+    } else {
+        const int64_t h = ((int64_t)blockIdx.x - light_blocks) * blockDim.x + threadIdx.x;
+        if (h >= n_heavy) return;
+        slot = heavy[h];
+    }
     Ke += set * st.a;
     if (Ks) Ks += set * st.b;
     vals += set * st.c;
-    const int64_t slot = t / 9;
-    const int k = (int)(t - 9 * slot);
-    const int i = k / 3, j = k - 3 * (k / 3);
-    double sum = 0.0;
-    for (int64_t c = cptr[slot]; c < cptr[slot + 1]; c++) {
-        const int64_t code = contrib[c];
-        const bool trans = code < 0;
-        const int64_t idx = trans ? ~code : code;
-        const int ki = trans ? j : i, kj = trans ? i : j;
-        if (idx < n_solid) {
-            sum += (double)Ke[9 * idx + 3 * ki + kj];
-        } else {
-            // surface block: idx - n_solid = (s * nps + a) * nps + b, packed with nps in the low bits
-            const int64_t sidx = idx - n_solid;
-            const int nps = (int)(sidx & 7);
-            const int64_t q = sidx >> 3;
-            const int64_t s = q / (nps * nps);
-            const int ab = (int)(q - s * nps * nps);
-            const int a = ab / nps, b = ab - nps * (ab / nps);
-            const int n = 3 * nps;
-            sum += Ks[s * n * n + (int64_t)(3 * a + ki) * n + 3 * b + kj];
-        }
-    }
+    double s[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) s[k] = 0.0;
+    const int64_t c1 = cptr[slot + 1];
+    for (int64_t c = cptr[slot]; c < c1; c++) add_block(s, contrib[c], Ke, Ks, n_solid, npe, flip);
     const int64_t r = blkrow[slot], col = colind[slot];
-    if (mval && i == j) sum += cm * mval[slot];
+    if (mval) {
+        const double m = cm * mval[slot];
+#pragma unroll
+        for (int i = 0; i < 3; i++) s[4 * i] += m;
+    }
     const uint8_t mr = bcmask ? bcmask[r] : 0, mc = bcmask ? bcmask[col] : 0;
-    if (((mr >> i) & 1) || ((mc >> j) & 1)) sum = (r == col && i == j) ? 1.0 : 0.0;
-    vals[t] = (VT)sum;
+    if (mr | mc) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                if (((mr >> i) & 1) || ((mc >> j) & 1)) s[3 * i + j] = (r == col && i == j) ? 1.0 : 0.0;
+    }
+    VT* out = vals + 9 * slot;
+#pragma unroll
+    for (int k = 0; k < 9; k++) out[k] = (VT)s[k];
 }
 
 // C = beta_R K + alpha M (Rayleigh damping) in place over K's gathered BSR values, with
@@ -166,28 +221,28 @@ void launch_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colin
                        bcmask, f);
 }
 
-void launch_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* contrib, const void* Ke, bool fp32,
-                          const double* Ks, int64_t n_solid_blocks, const int32_t* blkrow, const int32_t* colind,
-                          const double* mval, double cm, const uint8_t* bcmask, void* vals, bool vals_fp32,
-                          hipStream_t s, const SetStrides& b) {
-    if (nnzb == 0) return;
-    const dim3 grid((unsigned)((9 * nnzb + 255) / 256), b.n_sets);
+void launch_gather_matrix(const GatherMaps& g, const void* Ke, bool fp32, const double* Ks, const double* mval,
+                          double cm, const uint8_t* bcmask, void* vals, bool vals_fp32, bool flip, hipStream_t s,
+                          const SetStrides& b) {
+    if (g.nnzb == 0) return;
+    const int64_t light_blocks = (g.nnzb + 255) / 256;
+    const dim3 grid((unsigned)(light_blocks + (g.n_heavy + 255) / 256), b.n_sets);
+#define CMG_GATHER(KT, VT)                                                                                       \
+    hipLaunchKernelGGL((k_gather_matrix<KT, VT>), grid, dim3(256), 0, s, g.nnzb, g.cptr, g.contrib, g.heavy,      \
+                       g.n_heavy, light_blocks, (const KT*)Ke, Ks, g.n_solid, g.npe, g.blkrow, g.colind, mval, cm, \
+                       bcmask, (VT*)vals, flip, b)
     if (fp32) {
         if (vals_fp32)
-            hipLaunchKernelGGL((k_gather_matrix<float, float>), grid, dim3(256), 0, s, nnzb, cptr, contrib,
-                               (const float*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (float*)vals, b);
+            CMG_GATHER(float, float);
         else
-            hipLaunchKernelGGL((k_gather_matrix<float, double>), grid, dim3(256), 0, s, nnzb, cptr, contrib,
-                               (const float*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (double*)vals, b);
+            CMG_GATHER(float, double);
     } else {
         if (vals_fp32)
-            hipLaunchKernelGGL((k_gather_matrix<double, float>), grid, dim3(256), 0, s, nnzb, cptr, contrib,
-                               (const double*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask, (float*)vals, b);
+            CMG_GATHER(double, float);
         else
-            hipLaunchKernelGGL((k_gather_matrix<double, double>), grid, dim3(256), 0, s, nnzb, cptr, contrib,
-                               (const double*)Ke, Ks, n_solid_blocks, blkrow, colind, mval, cm, bcmask,
-                               (double*)vals, b);
+            CMG_GATHER(double, double);
     }
+#undef CMG_GATHER
 }
 
 }  // namespace cmg

@@ -169,9 +169,8 @@ void bgather_tangent(cmg_batch* b) {
     st.b = b->ptan ? c->S * 9 * c->nps * c->nps : 0;
     st.c = c->nnzb * 9;
     st.active = b->active.as<int32_t>();
-    launch_gather_matrix(c->nnzb, c->cptr.as<int64_t>(), c->contrib.as<int64_t>(), b->Ke.p, c->fp32,
-                         b->ptan ? b->Ks.as<double>() : c->Ks.as<double>(), c->n_solid_blocks, c->blkrow.as<int32_t>(), c->colind.as<int32_t>(),
-                         nullptr, 0.0, c->bcmask.as<uint8_t>(), b->vals.p, c->fp32, s, st);
+    launch_gather_matrix(gather_maps(c), b->Ke.p, c->fp32, b->ptan ? b->Ks.as<double>() : c->Ks.as<double>(), nullptr,
+                         0.0, c->bcmask.as<uint8_t>(), b->vals.p, c->fp32, false, s, st);
     SetStrides sj;
     sj.n_sets = b->S;
     sj.a = c->nnzb * 9;
@@ -434,7 +433,15 @@ int cmg_batch_create(cmg_ctx* c, int32_t n_sets, const cmg_material* materials, 
     const int64_t n = c->n;
     hipStream_t s = c->stream;
     std::vector<DevMaterial> dm((size_t)S * c->n_mats);
-    for (size_t i = 0; i < dm.size(); i++) CMG_TRY(to_dev_material(materials[i], dm[i]));
+    for (size_t i = 0; i < dm.size(); i++) {
+        // the batched solve is a conjugate gradient: it needs symmetric tangents, which the
+        // Holzapfel law as coded does not give (its isochoric factor is 1 and its switch
+        // enters the stress without its derivative, CBConstitutiveModelHolzapfel.cpp:89,121,148-154)
+        if (materials[i].law != CMG_LAW_GUCCIONE)
+            return fail("cmg_batch_create: the batched conjugate gradient needs symmetric tangents; only the "
+                        "Guccione law (symmetric with FromFunction, TanH or no active tension) is accepted");
+        CMG_TRY(to_dev_material(materials[i], dm[i]));
+    }
     CMG_TRY(upload(b->mats, dm.data(), dm.size(), s));
     const size_t vb = (size_t)S * n * sizeof(double);
     for (DevBuf* v : {&b->nodes, &b->d, &b->r, &b->rhs, &b->delta, &b->xe, &b->guess, &b->cg_r, &b->cg_z, &b->cg_p,

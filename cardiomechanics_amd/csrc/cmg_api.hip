@@ -348,24 +348,34 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
                     cnt[sl + 1]++;
                 }
         for (int64_t k = 0; k < nnzb; k++) cnt[k + 1] += cnt[k];
-        std::vector<int64_t> contrib(cnt[nnzb]);
+        if (ns_blocks + ((c->S * nps * nps) << 3) + 8 >= (int64_t)INT_MAX)
+            return fail("cmg_create: mesh too large for the int32 assembly maps");
+        std::vector<int32_t> contrib(cnt[nnzb]);
         std::vector<int64_t> fillp(cnt.begin(), cnt.end() - 1);
-        const bool trans = c->opt.transpose_tangent != 0;
+        // T10 blocks may enter transposed (the reference convention, CBElementSolidT10.cpp:559-565);
+        // the T4 finite-difference Jacobian is assembled as is (CBElementSolidT4.cpp:340-349)
+        const bool trans = c->opt.transpose_tangent != 0 && npe == 10;
+        c->maps_transposed = trans;
         for (int64_t e = 0; e < E; e++)
             for (int a = 0; a < npe; a++)
                 for (int b = 0; b < npe; b++) {
                     const int64_t sl = slot_of[(e * npe + a) * npe + b];
                     // transposed: block (a,b) of the assembled matrix takes (K_e(b,a))^T
-                    contrib[fillp[sl]++] = trans ? ~((e * npe + b) * npe + a) : ((e * npe + a) * npe + b);
+                    contrib[fillp[sl]++] = (int32_t)(trans ? ~((e * npe + b) * npe + a) : ((e * npe + a) * npe + b));
                 }
         for (int64_t si = 0; si < c->S; si++)
             for (int a = 0; a < nps; a++)
                 for (int b = 0; b < nps; b++) {
                     const int64_t sl = sslot[(si * nps + a) * nps + b];
-                    contrib[fillp[sl]++] = ns_blocks + ((((si * nps + a) * nps + b) << 3) | nps);
+                    contrib[fillp[sl]++] = (int32_t)(ns_blocks + ((((si * nps + a) * nps + b) << 3) | nps));
                 }
+        std::vector<int32_t> heavy;
+        for (int64_t k = 0; k < nnzb; k++)
+            if (cnt[k + 1] - cnt[k] > kHeavyContrib) heavy.push_back((int32_t)k);
+        c->n_heavy = (int64_t)heavy.size();
         CMG_TRY(upload(c->cptr, cnt.data(), cnt.size(), s));
         CMG_TRY(upload(c->contrib, contrib.data(), contrib.size(), s));
+        CMG_TRY(upload(c->heavy, heavy.data(), heavy.size(), s));
         // -- mass matrix: one scalar per block (consistent CBElementSolidT10.cpp:352-394 /
         //    CBElementSolidT4.cpp:365-410, or lumped :396-422 / :432-456)
         c->h_mval.assign(nnzb, 0.0);
@@ -488,6 +498,20 @@ ElemArgs elem_args(cmg_ctx* c, const double* x_dev, double time) {
     return a;
 }
 
+GatherMaps gather_maps(cmg_ctx* c) {
+    GatherMaps g;
+    g.nnzb = c->nnzb;
+    g.cptr = c->cptr.as<int64_t>();
+    g.contrib = c->contrib.as<int32_t>();
+    g.heavy = c->heavy.as<int32_t>();
+    g.n_heavy = c->n_heavy;
+    g.n_solid = (int32_t)c->n_solid_blocks;
+    g.npe = c->npe;
+    g.blkrow = c->blkrow.as<int32_t>();
+    g.colind = c->colind.as<int32_t>();
+    return g;
+}
+
 SurfArgs surf_args(cmg_ctx* c, const double* x_dev, bool with_volume) {
     SurfArgs a;
     a.S = c->S;
@@ -544,10 +568,8 @@ void gather_tangent(cmg_ctx* c, bool pressure, double cm, void* vals_dev) {
     const bool sp = pressure && c->S > 0 && c->opt.pressure_tangent;
     // without a pressure tangent the surface blocks read zeros: clear Ks
     if (c->S > 0 && !sp) launch_fill((int64_t)c->S * 9 * c->nps * c->nps, 0.0, c->Ks.as<double>(), c->stream);
-    launch_gather_matrix(c->nnzb, c->cptr.as<int64_t>(), c->contrib.as<int64_t>(), c->Ke.p, c->fp32,
-                         c->Ks.as<double>(), c->n_solid_blocks, c->blkrow.as<int32_t>(), c->colind.as<int32_t>(),
-                         cm != 0.0 ? c->mval.as<double>() : nullptr, cm, c->bcmask.as<uint8_t>(), vals_dev, c->fp32,
-                         c->stream);
+    launch_gather_matrix(gather_maps(c), c->Ke.p, c->fp32, c->Ks.as<double>(), cm != 0.0 ? c->mval.as<double>() : nullptr,
+                         cm, c->bcmask.as<uint8_t>(), vals_dev, c->fp32, false, c->stream);
 }
 
 // ||v||_2 (synchronising)
@@ -1258,9 +1280,12 @@ int build_damping(cmg_ctx* c, double time) {
         const int rc = eval_elements(c, c->nodes.as<double>(), time, true, false, &corrupt, true);
         if (rc != CMG_SUCCESS) return rc;
         if (c->S > 0) launch_fill((int64_t)c->S * 9 * c->nps * c->nps, 0.0, c->Ks.as<double>(), s);
-        launch_gather_matrix(c->nnzb, c->cptr.as<int64_t>(), c->contrib.as<int64_t>(), c->Ke.p, false,
-                             c->Ks.as<double>(), c->n_solid_blocks, c->blkrow.as<int32_t>(), c->colind.as<int32_t>(),
-                             nullptr, 0.0, c->bcmask.as<uint8_t>(), c->cvals.p, false, s);
+        // C is built from the tangent in the reference's layout: for T10 its finite-difference
+        // Jacobian is assembled transposed (CBElementSolidT10.cpp:559-565), so the reference's
+        // damping force is beta_R K^T w (CBSolverNewmarkBeta.cpp:657-668); Newton keeps K
+        const bool flip = c->npe == 10 && !c->maps_transposed;
+        launch_gather_matrix(gather_maps(c), c->Ke.p, false, c->Ks.as<double>(), nullptr, 0.0, c->bcmask.as<uint8_t>(),
+                             c->cvals.p, false, flip, s);
     }
     launch_rayleigh_finish(c->nnzb, c->blkrow.as<int32_t>(), c->colind.as<int32_t>(), c->bcmask.as<uint8_t>(),
                            c->mval.as<double>(), c->opt.rayleigh_beta, c->opt.rayleigh_alpha, c->cvals.as<double>(), s);
@@ -1328,7 +1353,9 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
         const double ta = now_s();
         launch_waxpby(n, 1.0, c->nodes.as<double>(), 1.0, d, xe, s);
         int corrupt = -1;
-        int rc = eval_elements(c, xe, time, true, true, &corrupt);
+        // residual only: the tangent is evaluated below once the iterate is accepted and a
+        // linear solve follows (not at the converged iterate, not on a backtracked one)
+        int rc = eval_elements(c, xe, time, false, true, &corrupt);
         if (rc != CMG_SUCCESS) {
             st->corrupt_element = corrupt;
             if (it > 0 && lambda > kMinDomainLambda) {
@@ -1409,6 +1436,12 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
         const double eta_k = std::min(0.5, std::max(eta, 0.5 * (c->opt.newton_atol + c->opt.newton_rtol * r0) / rnorm));
         rprev = rnorm;
         const double tb = now_s();
+        rc = eval_elements(c, xe, time, true, true, &corrupt);
+        if (rc != CMG_SUCCESS) {   // the residual kernel accepted this state: not expected
+            st->corrupt_element = corrupt;
+            st->converged_reason = -6;
+            break;
+        }
         gather_tangent(c, true, cm, c->vals.p);
         if (cd != 0.0 && c->damping) launch_add_scaled_values(c->nnzb, cd, c->cvals.as<double>(), c->vals.p, c->fp32, s);
         t_tan += now_s() - tb;

@@ -90,7 +90,9 @@ struct cmg_ctx {
     // surfaces
     cmg::DevBuf sconn, sgroup, pressure, fs, Ks, vol_partial;
     // pattern and maps
-    cmg::DevBuf rowptr, colind, blkrow, cptr, contrib, mval;
+    cmg::DevBuf rowptr, colind, blkrow, cptr, contrib, heavy, mval;
+    int64_t n_heavy = 0;
+    bool maps_transposed = false;   // T10 blocks enter the assembled tangent as K_e^T
     cmg::DevBuf inc_ptr, inc, sinc_ptr, sinc;
     // state and work vectors
     cmg::DevBuf Xref, nodes, xeval, d, dpred, vpred, vel, acc, r, delta, dsave, tmpA, tmpB;
@@ -123,4 +125,5 @@ namespace cmg {
 // kernel argument packs of a context for coordinates x_dev (device pointer)
 ElemArgs elem_args(cmg_ctx* c, const double* x_dev, double time);
 SurfArgs surf_args(cmg_ctx* c, const double* x_dev, bool with_volume);
+GatherMaps gather_maps(cmg_ctx* c);
 }  // namespace cmg

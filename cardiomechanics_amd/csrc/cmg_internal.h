@@ -94,11 +94,28 @@ void launch_gather_vector(int64_t N, const int64_t* inc_ptr, const int32_t* inc,
 // f += cm * M (d - dt) on free DOFs (Newmark inertia term); M stored as one scalar per block
 void launch_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval, double cm,
                           const double* d, const double* dpred, const uint8_t* bcmask, double* f, hipStream_t s);
-// BSR values from element/surface contributions + cm*M + identity on constrained DOFs
-void launch_gather_matrix(int64_t nnzb, const int64_t* cptr, const int64_t* contrib, const void* Ke, bool fp32,
-                          const double* Ks, int64_t n_solid_blocks, const int32_t* blkrow, const int32_t* colind,
-                          const double* mval, double cm, const uint8_t* bcmask, void* vals, bool vals_fp32,
-                          hipStream_t s, const SetStrides& b = SetStrides());   // strides: Ke, Ks, vals
+// Block-matrix assembly maps of a context.  Contributor codes (int32): a solid element
+// block (e * npe + a) * npe + b of K_e ([E][npe][npe][3][3]) or n_solid + a packed surface
+// block; ~code means the block enters transposed.  Slots with more than kHeavyContrib
+// contributors are also listed in `heavy` and gathered by their own threads.
+constexpr int kHeavyContrib = 8;
+struct GatherMaps {
+    int64_t nnzb = 0;
+    const int64_t* cptr = nullptr;     // [nnzb + 1]
+    const int32_t* contrib = nullptr;  // [cptr[nnzb]]
+    const int32_t* heavy = nullptr;    // [n_heavy] slots
+    int64_t n_heavy = 0;
+    int32_t n_solid = 0;               // E * npe * npe
+    int npe = 10;
+    const int32_t* blkrow = nullptr;   // [nnzb] block row of each slot
+    const int32_t* colind = nullptr;   // [nnzb]
+};
+// BSR values from element/surface contributions + cm*M + identity on constrained DOFs.
+// flip = 1 assembles every solid element block as K_e^T instead (the reference's T10
+// finite-difference layout when the maps hold the untransposed convention, and back).
+void launch_gather_matrix(const GatherMaps& g, const void* Ke, bool fp32, const double* Ks, const double* mval,
+                          double cm, const uint8_t* bcmask, void* vals, bool vals_fp32, bool flip, hipStream_t s,
+                          const SetStrides& b = SetStrides());   // strides: Ke, Ks, vals
 
 // Rayleigh damping C = beta_R K + alpha M in place over K's BSR values (constrained rows
 // and columns zeroed); vals += a C

@@ -107,7 +107,8 @@ typedef struct cmg_options {
                                    values, FP32 preconditioned Krylov operator; residuals,
                                    det J / penalty and convergence checks stay FP64 */
     int32_t transpose_tangent;  /* 1: assemble each T10 element tangent transposed, the
-                                   reference convention (CBElementSolidT10.cpp:559-565) */
+                                   reference convention (CBElementSolidT10.cpp:559-565); T4
+                                   tangents are never transposed (CBElementSolidT4.cpp:340-349) */
     int32_t pressure_tangent;   /* 0: none (the reference's T3 tangent is zero,
                                    CBElementSurfaceT3.cpp:268-269); 1: exact (default) */
     int32_t consistent_mass;    /* Solver.NewmarkBeta.ConsistentMassMatrix (default 1) */
@@ -134,9 +135,12 @@ typedef struct cmg_options {
     double rayleigh_beta;       /* Materials.Global.Damping.Rayleigh.Beta (s, default 0): Newmark steps
                                    add C (v~ + gamma/(beta dt) (d - d~)) to the residual and
                                    gamma/(beta dt) C to the tangent, C = alpha M + beta_R K(x_n) with
-                                   K the element tangent at the step's start configuration, rebuilt
-                                   whenever that configuration changes (CBSolverNewmarkBeta.cpp:
-                                   384-390, 622-626, 657-702, 857-866); off when both are 0 */
+                                   K the element tangent at the step's start configuration in the
+                                   reference's finite-difference layout (K_e^T per T10 element,
+                                   CBElementSolidT10.cpp:559-565; K_e per T4) whatever
+                                   transpose_tangent says, rebuilt whenever that configuration
+                                   changes (CBSolverNewmarkBeta.cpp:384-390, 622-626, 657-702,
+                                   857-866); off when both are 0 */
     double reserved[2];
 } cmg_options;
 
@@ -317,7 +321,8 @@ typedef struct cmg_batch_info {
     double tune_seconds;
 } cmg_batch_info;
 
-/* materials: [n_sets][n_materials of ctx], same order as at cmg_create.  The batch
+/* materials: [n_sets][n_materials of ctx], same order as at cmg_create; Guccione only
+ * (CMG_ERR_INVALID otherwise: the Holzapfel tangent as coded is unsymmetric).  The batch
  * shares ctx's mesh, pattern and pressures (cmg_set_pressures on ctx); ctx must
  * outlive it.  Every set starts from ctx's current node coordinates. */
 int cmg_batch_create(cmg_ctx* ctx, int32_t n_sets, const cmg_material* materials, cmg_batch** out);

@@ -133,11 +133,17 @@ class CpuMechanics:
     # -- Newton -----------------------------------------------------------------
     def damping_matrix(self, time):
         """C = alpha M + beta_R K(x_n): K the element tangent without surface terms at
-        the current nodes and the given time; constrained rows and columns zero."""
+        the current nodes and the given time, in the layout the reference assembles its
+        finite-difference Jacobian in (CBSolverNewmarkBeta.cpp:657-668 builds C from
+        it): transposed per T10 element (CBElementSolidT10.cpp:559-565, row = perturbed
+        DOF), as is for T4 (CBElementSolidT4.cpp:340-349); constrained rows and columns
+        zero."""
         alpha, beta_r = self.rayleigh
         C = alpha * self.M
         if beta_r != 0.0:
             Ke = self.element_tangents(self.nodes, time)
+            if self.mesh.npe == 10:
+                Ke = np.ascontiguousarray(Ke.transpose(0, 2, 1))
             nd = Ke.shape[1]
             K = sp.csr_matrix((Ke.ravel(), (np.repeat(self.dofs, nd, axis=1).ravel(),
                                             np.tile(self.dofs, (1, nd)).ravel())), shape=(self.n, self.n))

@@ -518,3 +518,42 @@ def test_element_data_does_not_alias_the_mesh_constraints():
     ed = O.ElementData(mesh)
     ed.bcmask[:] = 0
     assert np.array_equal(mesh.bc, bc)
+
+
+def test_damping_matrix_uses_the_reference_layout_of_the_tangent():
+    """The reference builds C = alpha M + beta_R K from its finite-difference Jacobian
+    (CBSolverNewmarkBeta.cpp:657-668), which T10 assembles with row = perturbed DOF
+    (CBElementSolidT10.cpp:559-565), i.e. K^T.  The oracle's C applied to w must equal
+    alpha M w + beta_R (assembled reference-layout FD Jacobian) w, to finite-difference
+    noise, on Holzapfel elements with active tension, whose tangent as coded is
+    unsymmetric: the untransposed K misses by far more than that noise."""
+    import scipy.sparse as sp
+    from oracle.drivers import CpuMechanics
+
+    mesh, mats = M.lv_mesh(dims=(2, 4, 8)), [MT.lv_holzapfel()]
+    alpha, beta_r = 500.0, 0.01
+    cpu = CpuMechanics(mesh, mats, rayleigh=(alpha, beta_r))
+    L = np.ptp(mesh.coords, axis=0).max()
+    cpu.nodes = (mesh.coords + M.smooth_displacement(mesh.coords, L, 0.02, 1234)).ravel()
+    t = 0.12
+    C = cpu.damping_matrix(t)
+    ed = O.ElementData(mesh)
+    rc, Jfd, _ = O.tangent_batch(ed, cpu.nodes.reshape(-1, 3), mats, t, kind="fd", eps=1e-9)
+    assert rc == O.SUCCESS
+    nd = Jfd.shape[1]
+    rows = np.repeat(cpu.dofs, nd, axis=1).ravel()
+    cols = np.tile(cpu.dofs, (1, nd)).ravel()
+    A_ref = sp.csr_matrix((Jfd.ravel(), (rows, cols)), shape=(cpu.n, cpu.n))
+    rc, Kcs, _ = O.tangent_batch(ed, cpu.nodes.reshape(-1, 3), mats, t, kind="cs")
+    K = sp.csr_matrix((Kcs.ravel(), (rows, cols)), shape=(cpu.n, cpu.n))
+    D = sp.diags((~cpu.bcdof).astype(float))
+    w = np.random.default_rng(11).normal(size=cpu.n)
+    want = D @ (alpha * (cpu.M @ (D @ w)) + beta_r * (A_ref @ (D @ w)))
+    got = C @ w
+    wrong = D @ (alpha * (cpu.M @ (D @ w)) + beta_r * (K @ (D @ w)))
+    scale = np.linalg.norm(beta_r * (A_ref @ (D @ w)))
+    err = np.linalg.norm(got - want) / scale
+    miss = np.linalg.norm(wrong - want) / scale
+    print(f"C w vs reference layout: {err:.2e}; untransposed K: {miss:.2e}")
+    assert err <= 1e-5
+    assert miss >= 1e-3
