@@ -25,7 +25,7 @@ EXPORTED = [
     "cmg_mass_matrix", "cmg_element_export", "cmg_spmv_tuning", "cmg_residual_device", "cmg_residual_tangent_device", "cmg_element_kernel_device",
     "cmg_spmv", "cmg_spmv_layout", "cmg_linear_solve", "cmg_set_state", "cmg_get_state", "cmg_set_initial_guess",
     "cmg_static_step", "cmg_newmark_step", "cmg_newmark_energies", "cmg_newmark_solve", "cmg_trial_volumes", "cmg_commit_step",
-    "cmg_copy_device",
+    "cmg_copy_device", "cmg_residual_device_noalloc",
     "cmg_batch_create", "cmg_batch_destroy", "cmg_batch_configure", "cmg_batch_info_get", "cmg_batch_static_step",
     "cmg_batch_get_nodes", "cmg_batch_set_nodes", "cmg_batch_residual", "cmg_batch_cavity_volumes",
     "cmg_mlp_num_weights", "cmg_mlp_create", "cmg_mlp_destroy", "cmg_mlp_get_weights", "cmg_mlp_set_weights",
@@ -127,6 +127,7 @@ def load():
         "cmg_element_export": (ctypes.c_int, [P, P, P]),
         "cmg_spmv_tuning": (ctypes.c_int, [P, P, P, P]),
         "cmg_residual_device": (ctypes.c_int, [P, P, dbl, i32, P]),
+        "cmg_residual_device_noalloc": (ctypes.c_int, [P, P, dbl, i32, P]),
         "cmg_residual_tangent_device": (ctypes.c_int, [P, P, dbl, i32, P, P]),
         "cmg_element_kernel_device": (ctypes.c_int, [P, P, dbl, i32]),
         "cmg_spmv": (ctypes.c_int, [P, P, P, P]),

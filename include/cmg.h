@@ -238,6 +238,17 @@ int cmg_residual_tangent_device(cmg_ctx* ctx, const double* x_dev, double time, 
  * element buffers (the unit measured as element residual+tangent throughput) */
 int cmg_element_kernel_device(cmg_ctx* ctx, const double* x_dev, double time, int32_t with_tangent);
 
+/* No counterpart in the reference: cmg_residual_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and result as cmg_residual_device, but its scratch (the
+ * corrupt-element flag, the element forces [E][3 npe] and, with pressure, the surface
+ * forces [S][3 nps]) is one fixed 64 MiB device array per GPU, shared by every call on
+ * that GPU: meshes whose forces need more (E*3*npe + S*3*nps > 8 Mi - 32 doubles, about
+ * 279k T10 elements) are rejected with CMG_ERR_INVALID, and calls that overlap on one GPU
+ * race.  Synchronises the context's stream for the corrupt-element check. */
+int cmg_residual_device_noalloc(cmg_ctx* ctx, const double* x_dev, double time, int32_t include_pressure,
+                                double* f_dev);
+
 /* Device copy of n doubles on the context's stream (a plain streaming kernel; used as
  * the known-size reference when calibrating HBM traffic counters). */
 int cmg_copy_device(cmg_ctx* ctx, const double* src_dev, double* dst_dev, int64_t n);
