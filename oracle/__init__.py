@@ -75,6 +75,8 @@ def lib():
             "orc_tanh_tension": (ctypes.c_double, [_M, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "orc_active_stress_model": (None, [_M, _D, ctypes.c_double, ctypes.c_double, _D]),
             "orc_t10_shape": (None, [_D, _D, _D]),
+            "orc_shape_batch": (None, [ctypes.c_int32, ctypes.c_int64, _I32, _D, _D, _D]),
+            "orc_mass_batch": (None, [ctypes.c_int32, ctypes.c_int64, _I32, _D, _D, _U8, ctypes.c_int32, _D]),
             "orc_t10_residual": (ctypes.c_int32, [_D, _D, _D, ctypes.c_double, _U8, _M, ctypes.c_double,
                                                   ctypes.c_double, _D]),
             "orc_t10_fd_tangent": (ctypes.c_int32, [_D, _D, _D, ctypes.c_double, _U8, _M, ctypes.c_double,
@@ -351,25 +353,35 @@ class ElementData:
 
     def __init__(self, mesh):
         E, npe = mesh.n_elements, mesh.npe
-        X = mesh.coords[mesh.elements].reshape(E, 3 * npe)
+        self.conn = np.ascontiguousarray(mesh.elements, dtype=np.int32)
+        coords, pc = _d(mesh.coords)
+        self.dNdX = np.empty((E, 150 if npe == 10 else 12))
+        d = np.empty(E)
+        # t10_shape / t4_shape of every element (OpenMP)
+        lib().orc_shape_batch(npe, E, self.conn.ctypes.data_as(_I32), pc, self.dNdX.ctypes.data_as(_D),
+                              d.ctypes.data_as(_D))
         if npe == 10:
-            self.dNdX = np.empty((E, 150))
-            self.w = np.empty(E)
-            for e in range(E):
-                self.dNdX[e], self.w[e] = t10_shape(X[e])
+            self.w = d
             self.bases = np.ascontiguousarray(mesh.bases.reshape(E, 45))
         else:
-            self.dNdX = np.empty((E, 12))
-            d = np.empty(E)
-            for e in range(E):
-                self.dNdX[e], d[e] = t4_shape(X[e])
             self.w = d / 6.0
             self.bases = np.ascontiguousarray(mesh.bases.reshape(E, 9))
         self.npe = npe
-        self.conn = np.ascontiguousarray(mesh.elements, dtype=np.int32)
         self.bcmask = np.array(mesh.bc, dtype=np.uint8)          # a copy: callers may edit it
         self.matidx = np.ascontiguousarray(mesh.material, dtype=np.int32)
         self.tact = np.ascontiguousarray(mesh.activation, dtype=np.float64)
+
+
+def mass_batch(ed: ElementData, density, consistent=True):
+    """element mass matrices [E][3npe][3npe] (t10_mass_consistent / t10_mass_lumped /
+    t4_mass_consistent of every element); density per element"""
+    E = ed.conn.shape[0]
+    nd = 3 * ed.npe
+    Me = np.zeros((E, nd, nd))
+    rho, prho = _d(density)
+    lib().orc_mass_batch(ed.npe, E, ed.conn.ctypes.data_as(_I32), ed.w.ctypes.data_as(_D), prho,
+                         ed.bcmask.ctypes.data_as(_U8), int(consistent), Me.ctypes.data_as(_D))
+    return Me
 
 
 def residual_batch(ed: ElementData, coords, mats, time=0.0):

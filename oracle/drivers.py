@@ -58,21 +58,35 @@ class CpuMechanics:
         self.guess = np.zeros(self.n)
 
     # -- element / surface loops ------------------------------------------------
-    def _mass(self, consistent):
-        rows, cols, vals = [], [], []
-        for e in range(self.mesh.n_elements):
-            bc = self.bcdof[self.dofs[e]].astype(np.uint8)
-            rho = self.materials[self.mesh.material[e]].density
-            if self.mesh.npe == 10:
-                detJ = self.ed.w[e]
-                Me = O.t10_mass_consistent(detJ, rho, bc) if consistent else O.t10_mass_lumped(detJ / 6, rho, bc)
-            else:
-                Me = O.t4_mass_consistent(self.ed.w[e], rho, bc)
-            rows.append(np.repeat(self.dofs[e], Me.shape[0]))
-            cols.append(np.tile(self.dofs[e], Me.shape[0]))
-            vals.append(Me.ravel())
-        return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                             shape=(self.n, self.n))
+    def _mass(self, consistent, chunk=50_000):
+        """Assembled mass matrix from the oracle's element mass matrices (t10_mass_consistent
+        / t10_mass_lumped / t4_mass_consistent).  Each is m_ab (x) I3 with the rows of
+        constrained DOFs zeroed, so the nodal scalars m_ab of the unconstrained element
+        matrices are assembled (in chunks of elements) and the constrained rows cleared
+        once: M = D_free (m (x) I3)."""
+        rho = np.array([self.materials[k].density for k in self.mesh.material], dtype=np.float64)
+        if self.mesh.npe == 4:
+            consistent = True          # the T4 mass restated here is the consistent one
+        free_ed = O.ElementData.__new__(O.ElementData)
+        free_ed.__dict__.update(self.ed.__dict__)
+        free_ed.bcmask = np.zeros_like(self.ed.bcmask)
+        E, npe = self.mesh.n_elements, self.mesh.npe
+        m_node = sp.csr_matrix((self.N, self.N))
+        for lo in range(0, E, chunk):
+            hi = min(lo + chunk, E)
+            sub = O.ElementData.__new__(O.ElementData)
+            sub.__dict__.update(free_ed.__dict__)
+            sub.conn = np.ascontiguousarray(free_ed.conn[lo:hi])
+            sub.w = np.ascontiguousarray(free_ed.w[lo:hi])
+            Me = O.mass_batch(sub, rho[lo:hi], consistent).reshape(hi - lo, npe, 3, npe, 3)
+            m = Me[:, :, 0, :, 0]
+            if lo == 0:   # the element matrices are m_ab (x) I3
+                assert np.array_equal(Me, m[:, :, None, :, None] * np.eye(3)[None, None, :, None, :])
+            rows = np.repeat(sub.conn, npe, axis=1).ravel()
+            cols = np.tile(sub.conn, (1, npe)).ravel()
+            m_node = m_node + sp.csr_matrix((m.ravel(), (rows, cols)), shape=(self.N, self.N))
+        M = sp.kron(m_node, sp.eye(3), format="csr")
+        return (sp.diags((~self.bcdof).astype(float)) @ M).tocsr()
 
     def surface_forces(self, x):
         f = np.zeros(self.n)
@@ -88,8 +102,7 @@ class CpuMechanics:
         rc, fe, st = O.residual_batch(self.ed, x.reshape(-1, 3), self.materials, time)
         if rc != O.SUCCESS:
             raise FloatingPointError(f"corrupt element {int(np.argmax(st != O.SUCCESS))}")
-        f = np.zeros(self.n)
-        np.add.at(f, self.dofs.ravel(), fe.ravel())
+        f = np.bincount(self.dofs.ravel(), weights=fe.ravel(), minlength=self.n)   # in element order
         if include_pressure:
             f += self.surface_forces(x)
         f[self.bcdof] = 0.0

@@ -983,6 +983,44 @@ int32_t orc_cs_tangent_batch(int32_t npe, int64_t n_elem, const int32_t* conn, c
     return bad ? ORC_FAILED : ORC_SUCCESS;
 }
 
+// shape-function derivatives of every element (T10: orc_t10_shape; T4: orc_t4_shape),
+// dNdX [E][150] / [E][12] and detJ [E] (CBElementSolid*::CalcShapeFunctionDerivatives)
+void orc_shape_batch(int32_t npe, int64_t n_elem, const int32_t* conn, const double* coords, double* dNdX,
+                     double* detJ) {
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < n_elem; e++) {
+        double X[30];
+        for (int a = 0; a < npe; a++)
+            for (int c = 0; c < 3; c++) X[3 * a + c] = coords[3 * (int64_t)conn[e * npe + a] + c];
+        if (npe == 10)
+            orc_t10_shape(X, dNdX + 150 * e, detJ + e);
+        else
+            orc_t4_shape(X, dNdX + 12 * e, detJ + e);
+    }
+}
+
+// element mass matrices of every element: T10 consistent (w = detJ) or lumped (w = detJ,
+// volume detJ/6), T4 consistent (w = volume); rows of constrained DOFs zero
+void orc_mass_batch(int32_t npe, int64_t n_elem, const int32_t* conn, const double* w, const double* density,
+                    const uint8_t* bcmask, int32_t consistent, double* Me) {
+    const int nd = 3 * npe;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < n_elem; e++) {
+        uint8_t bc[30];
+        for (int a = 0; a < npe; a++)
+            for (int c = 0; c < 3; c++) bc[3 * a + c] = (bcmask[conn[e * npe + a]] >> c) & 1;
+        double* out = Me + (int64_t)nd * nd * e;
+        if (npe == 10) {
+            if (consistent)
+                orc_t10_mass_consistent(w[e], density[e], bc, out);
+            else
+                orc_t10_mass_lumped(w[e] / 6, density[e], bc, out);
+        } else {
+            orc_t4_mass_consistent(w[e], density[e], bc, out);
+        }
+    }
+}
+
 int32_t orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();
