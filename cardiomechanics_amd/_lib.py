@@ -59,7 +59,8 @@ class CmgOptions(ctypes.Structure):
                 ("newton_stol", ctypes.c_double), ("krylov_rtol", ctypes.c_double),
                 ("newmark_beta", ctypes.c_double), ("newmark_gamma", ctypes.c_double),
                 ("fd_epsilon", ctypes.c_double), ("rayleigh_alpha", ctypes.c_double),
-                ("rayleigh_beta", ctypes.c_double), ("reserved", ctypes.c_double * 2)]
+                ("rayleigh_beta", ctypes.c_double), ("preconditioner", ctypes.c_int32),
+                ("reserved_i", ctypes.c_int32), ("krylov_eta_max", ctypes.c_double)]
 
 
 class CmgSolveStats(ctypes.Structure):

@@ -5,6 +5,8 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -148,6 +150,8 @@ void cmg_default_options(cmg_options* o) {
     o->newmark_beta = 0.25;
     o->newmark_gamma = 0.5;
     o->fd_epsilon = 1e-12;
+    o->preconditioner = 0;   // vertex-patch Schwarz for T10, block Jacobi otherwise
+    o->krylov_eta_max = 0.5;
 }
 
 const char* cmg_last_error(void) { return g_last_error.c_str(); }
@@ -203,6 +207,10 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     c->nps = c->S > 0 ? mesh->nodes_per_surface : 0;
     c->n_groups = std::max(1, mesh->n_groups);
     c->fp32 = c->opt.precision == CMG_PREC_MIXED;
+    {
+        const char* tr = std::getenv("CMG_NEWTON_TRACE");   // per-iteration Newton log on stderr
+        c->trace = tr && *tr && *tr != '0';
+    }
     c->damping = c->opt.rayleigh_alpha != 0.0 || c->opt.rayleigh_beta != 0.0;
     const int64_t N = c->N, E = c->E;
     // -- validate connectivity
@@ -396,6 +404,8 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
         }
         CMG_TRY(upload(c->mval, c->h_mval.data(), nnzb, s));
     }
+    // -- preconditioner patches (need the block pattern)
+    if (c->opt.preconditioner != 1) CMG_TRY(build_vertex_patches(c.get(), mesh->elements));
     // -- surfaces
     c->h_pressure.assign(c->n_groups, 0.0);
     CMG_TRY(upload(c->pressure, c->h_pressure.data(), c->n_groups, s));
@@ -605,8 +615,29 @@ int to_device_coeffs(cmg_ctx* c, const double* h, int k) {
     return CMG_SUCCESS;
 }
 
-// GMRES(m) with right block-Jacobi preconditioning: solves A x = b (x holds the
-// initial guess).  Classical Gram-Schmidt with DGKS re-orthogonalisation.
+// the Krylov preconditioner: vertex-patch additive Schwarz when the context built its
+// patches (not for the device-scalar BiCGStab, whose fused kernels apply block Jacobi),
+// 3x3 nodal block Jacobi otherwise
+bool use_schwarz(const cmg_ctx* c) { return c->schwarz && c->opt.krylov_method != 1; }
+
+void precond_setup(cmg_ctx* c, const void* vals) {
+    if (use_schwarz(c))
+        launch_patch_setup(c, vals);
+    else
+        launch_block_jacobi_setup(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32,
+                                  c->dinv.as<double>(), c->stream);
+}
+
+void precond_apply(cmg_ctx* c, const double* r, double* z) {
+    if (use_schwarz(c))
+        launch_patch_apply(c, r, z);
+    else
+        launch_block_jacobi_apply(c->N, c->dinv.as<double>(), r, z, c->stream);
+}
+
+// GMRES(m) with right preconditioning (block Jacobi or vertex-patch Schwarz): solves
+// A x = b (x holds the initial guess).  Classical Gram-Schmidt with DGKS
+// re-orthogonalisation.
 int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol, int maxit, cmg_solve_stats* st) {
     hipStream_t s = c->stream;
     const int64_t n = c->n;
@@ -614,8 +645,7 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
     double* V = c->gm_V.as<double>();
     double* w = c->tmpA.as<double>();
     double* z = c->tmpB.as<double>();
-    launch_block_jacobi_setup(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32,
-                              c->dinv.as<double>(), s);
+    precond_setup(c, vals);
     double bnorm;
     CMG_TRY(norm2(c, b, &bnorm));
     // r = b - A x
@@ -644,7 +674,7 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
         for (; j < m && total < maxit;) {
             double* vj = V + (int64_t)j * n;
             double* vn = V + (int64_t)(j + 1) * n;
-            launch_block_jacobi_apply(c->N, c->dinv.as<double>(), vj, z, s);
+            precond_apply(c, vj, z);
             launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, z, vn, s, c->spmv_lanes,
                         c->spmv_layout);
             // pass 1: h_i = v_i . w (i <= j) and ||w||^2 in one multi-dot
@@ -698,7 +728,7 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
         launch_fill(n, 0.0, w, s);
         CMG_TRY(to_device_coeffs(c, ny.data(), j));
         launch_multiaxpy(n, j, V, n, c->gm_y.as<double>(), w, s);   // w = V y
-        launch_block_jacobi_apply(c->N, c->dinv.as<double>(), w, z, s);
+        precond_apply(c, w, z);
         launch_axpby(n, 1.0, z, 1.0, x, s);
         // true residual
         launch_spmv(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), vals, c->fp32, x, w, s, c->spmv_lanes,
@@ -718,7 +748,7 @@ int gmres(cmg_ctx* c, const void* vals, const double* b, double* x, double rtol,
 }
 
 
-// BiCGStab with right block-Jacobi preconditioning (van der Vorst 1992).  Two host
+// BiCGStab with right preconditioning (van der Vorst 1992).  Two host
 // synchronisations per iteration: (r^, v) for alpha, and one fused fetch of
 // (r^,t) (s,t) (t,t) (r^,s) (s,s) from which omega, the next rho and the updated
 // residual norm ||s - omega t|| follow without another pass.
@@ -754,7 +784,8 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
             for (int j = 0; j < k; j++) out[j] += c->h_part[(int64_t)blk * k + j];
         return CMG_SUCCESS;
     };
-    launch_block_jacobi_setup(c->N, rp, ci, vals, c->fp32, c->dinv.as<double>(), s);
+    const bool sw = use_schwarz(c);
+    precond_setup(c, vals);
     double bnorm;
     CMG_TRY(norm2(c, b, &bnorm));
     launch_spmv(c->N, rp, ci, vals, c->fp32, x, v, s, c->spmv_lanes, c->spmv_layout);
@@ -782,7 +813,8 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
         } else {
             beta = (rho_new / rho) * (alpha / omega);
         }
-        launch_bicg_p_bj(c->N, fresh, beta, omega, r, v, dinv, p, ph, s);
+        launch_bicg_p_bj(c->N, fresh, beta, omega, r, v, sw ? nullptr : dinv, p, ph, s);
+        if (sw) precond_apply(c, p, ph);
         fresh = false;
         rho = rho_new;
         launch_spmv(c->N, rp, ci, vals, c->fp32, ph, v, s, c->spmv_lanes, c->spmv_layout);
@@ -792,7 +824,8 @@ int bicgstab(cmg_ctx* c, const void* vals, const double* b, double* x, double rt
         bool breakdown = !(std::fabs(d0) > 0.0) || !std::isfinite(d0);
         if (!breakdown) {
             alpha = rho / d0;
-            launch_bicg_s_bj(c->N, alpha, r, v, dinv, sv, sh, s);   // s = r - alpha v, sh = D^-1 s
+            launch_bicg_s_bj(c->N, alpha, r, v, sw ? nullptr : dinv, sv, sh, s);   // s = r - alpha v, sh = D^-1 s
+            if (sw) precond_apply(c, sv, sh);
             launch_spmv(c->N, rp, ci, vals, c->fp32, sh, t, s, c->spmv_lanes, c->spmv_layout);
             launch_bicg_dots5(n, rh, sv, t, part, s);              // (rh,t) (s,t) (t,t) (rh,s) (s,s)
             double d5[5];
@@ -1358,6 +1391,7 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
         int rc = eval_elements(c, xe, time, false, true, &corrupt);
         if (rc != CMG_SUCCESS) {
             st->corrupt_element = corrupt;
+            if (c->trace) std::fprintf(stderr, "[newton]   element %d inverted at lambda=%.4g\n", corrupt, lambda);
             if (it > 0 && lambda > kMinDomainLambda) {
                 // step into an inverted state: backtrack (SNESSetFunctionDomainError analogue)
                 lambda *= 0.5;
@@ -1401,6 +1435,7 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
             st->initial_residual_norm = r0;
         } else if (rnorm > rprev && lambda > 1.0 / 64 && rnorm > c->opt.newton_atol) {
             // no decrease: backtrack along the last Newton direction
+            if (c->trace) std::fprintf(stderr, "[newton]   backtrack lambda=%.4g |r|=%.3e\n", lambda, rnorm);
             lambda *= 0.5;
             launch_axpby(n, -lambda, delta, 1.0, d, s);
             it--;
@@ -1429,11 +1464,14 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
             st->converged_reason = -5;   // SNES_DIVERGED_MAX_IT
             break;
         }
-        if (it > 0) eta = std::min(0.5, std::max(c->opt.krylov_rtol, 0.9 * (rnorm / rprev) * (rnorm / rprev)));
+        const double eta_max = c->opt.krylov_eta_max > 0.0 ? c->opt.krylov_eta_max : 0.5;
+        if (it == 0) eta = std::min(eta, eta_max);
+        if (it > 0) eta = std::min(eta_max, std::max(c->opt.krylov_rtol, 0.9 * (rnorm / rprev) * (rnorm / rprev)));
         // no oversolving: a linear residual below half the nonlinear tolerance is all the
         // next iterate needs (Kelley 1995, eq. 6.20: eta >= 0.5 tau_t / ||r||, tau_t =
         // atol + rtol ||r_0||)
-        const double eta_k = std::min(0.5, std::max(eta, 0.5 * (c->opt.newton_atol + c->opt.newton_rtol * r0) / rnorm));
+        const double eta_k =
+            std::min(eta_max, std::max(eta, 0.5 * (c->opt.newton_atol + c->opt.newton_rtol * r0) / rnorm));
         rprev = rnorm;
         const double tb = now_s();
         rc = eval_elements(c, xe, time, true, true, &corrupt);
@@ -1452,6 +1490,9 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
         CMG_TRY(krylov(c, c->vals.p, c->dsave.as<double>(), delta, eta_k, c->opt.krylov_maxit, &ls));
         kry_total += ls.iterations;
         t_kry += now_s() - tk;
+        if (c->trace)
+            std::fprintf(stderr, "[newton] t=%.6g it=%d |r|=%.3e eta=%.2e krylov=%d lin.res=%.2e/%.2e\n", time, it,
+                         rnorm, eta_k, ls.iterations, ls.final_residual, ls.initial_residual);
         lambda = 1.0;
         launch_axpby(n, 1.0, delta, 1.0, d, s);
         CMG_TRY(norm2(c, delta, &snorm));

@@ -75,6 +75,7 @@ struct cmg_ctx {
     bool bicg_use_graph = true;
     double spmv_ms[9] = {};     // timed SpMV: block at 4/8/16 lanes, chunks at 4/8/16, flat at 16/32/64
     bool fp32 = false;
+    bool trace = false;         // CMG_NEWTON_TRACE: one stderr line per Newton iteration
     // host copies needed for maps / exports
     std::vector<uint8_t> h_bc;
     std::vector<int64_t> h_rowptr;
@@ -119,6 +120,13 @@ struct cmg_ctx {
     bool trial_pending = false;
     double trial_time = 0.0, trial_dt = 0.0;
     cmg::DevBuf override_guess;
+    // vertex-patch additive Schwarz preconditioner (schwarz.hip): patch node lists, slot
+    // maps into the block pattern, inverse offsets, node -> patch-row positions, the
+    // inverses and the per-patch results
+    bool schwarz = false;
+    int64_t sw_P = 0;
+    int sw_nd_max = 0;
+    cmg::DevBuf sw_pptr, sw_pnodes, sw_sptr, sw_slots, sw_iptr, sw_cptr, sw_cpos, sw_inv, sw_y;
 };
 
 namespace cmg {
@@ -126,4 +134,10 @@ namespace cmg {
 ElemArgs elem_args(cmg_ctx* c, const double* x_dev, double time);
 SurfArgs surf_args(cmg_ctx* c, const double* x_dev, bool with_volume);
 GatherMaps gather_maps(cmg_ctx* c);
+// vertex-patch additive Schwarz (schwarz.hip); patches of more than kMaxPatchDofs DOFs
+// leave the context with block Jacobi
+constexpr int kMaxPatchDofs = 72;
+int build_vertex_patches(cmg_ctx* c, const int32_t* elements);   // sets c->schwarz
+void launch_patch_setup(cmg_ctx* c, const void* vals);            // inverses of the patch matrices
+void launch_patch_apply(cmg_ctx* c, const double* r, double* z);  // z = sum_p R_p^T A_p^-1 R_p r
 }  // namespace cmg

@@ -271,6 +271,7 @@ __global__ __launch_bounds__(256) void k_bicg_p_bj(int64_t N, int fresh, double 
         q[c] = fresh ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
         p[i] = q[c];
     }
+    if (!dinv) return;   // another preconditioner follows
     const double* a = dinv + 9 * n;
     ph[3 * n] = a[0] * q[0] + a[1] * q[1] + a[2] * q[2];
     ph[3 * n + 1] = a[3] * q[0] + a[4] * q[1] + a[5] * q[2];
@@ -289,6 +290,7 @@ __global__ __launch_bounds__(256) void k_bicg_s_bj(int64_t N, double alpha, cons
         q[c] = r[i] - alpha * v[i];
         sv[i] = q[c];
     }
+    if (!dinv) return;   // another preconditioner follows
     const double* a = dinv + 9 * n;
     sh[3 * n] = a[0] * q[0] + a[1] * q[1] + a[2] * q[2];
     sh[3 * n + 1] = a[3] * q[0] + a[4] * q[1] + a[5] * q[2];

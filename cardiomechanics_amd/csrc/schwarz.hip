@@ -1,0 +1,260 @@
+// Vertex-patch additive Schwarz preconditioner for T10 tangents (a replacement of the
+// 3x3 nodal block Jacobi inside the Krylov solves; the reference factorises the whole
+// matrix instead, CBSolverNewmarkBeta.cpp:176-192).
+//
+// Patch p belongs to element vertex v: the nodes {v} and the midside nodes of every edge
+// incident to v (every midside node is in the patches of its two end vertices, every
+// vertex in its own).  z = sum_p R_p^T A_p^{-1} R_p r with A_p the principal submatrix
+// of the assembled tangent on the patch's DOFs.  On the near-incompressible wall the
+// patches carry the locally divergence-free motions that point block Jacobi cannot
+// (about 3x fewer BiCGStab iterations on the LV meshes).
+//
+// Device work per tangent: one wave per patch gathers A_p from the block values into
+// LDS and inverts it by Gauss-Jordan elimination with partial pivoting (FP64), storing
+// the inverse column-major (FP64, or FP32 in mixed precision).  Per application: one
+// wave per patch forms y_p = A_p^{-1} r_p (lane i owns row i: coalesced column reads),
+// then every node sums its one or two patch results in patch order (deterministic,
+// no atomics).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cmg_ctx.h"
+
+namespace cmg {
+namespace {
+
+constexpr int kWave = 64;
+
+// A_p^{-1} of every patch.  Dynamic LDS: nd_max^2 + nd_max doubles + nd_max ints.
+template <typename VT, typename IT>
+__global__ __launch_bounds__(kWave) void k_patch_setup(int64_t P, const int32_t* pptr, const int32_t* pnodes,
+                                                       const int64_t* sptr, const int32_t* slots, const int64_t* iptr,
+                                                       const VT* vals, IT* inv, int nd_max) {
+    extern __shared__ double lds[];
+    const int64_t p = blockIdx.x;
+    if (p >= P) return;
+    const int lane = threadIdx.x;
+    const int np = pptr[p + 1] - pptr[p];
+    const int nd = 3 * np;
+    double* A = lds;
+    double* col = lds + (size_t)nd_max * nd_max;
+    int* perm = reinterpret_cast<int*>(col + nd_max);
+    const int32_t* sl = slots + sptr[p];
+    for (int e = lane; e < nd * nd; e += kWave) {
+        const int I = e / nd, J = e - (e / nd) * nd;
+        const int s = sl[(I / 3) * np + J / 3];
+        A[e] = s >= 0 ? (double)vals[9 * (int64_t)s + 3 * (I % 3) + (J % 3)] : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < nd; k++) {
+        // pivot: the largest |A[i][k]|, i >= k (smallest i among equals)
+        double best = -1.0;
+        int bi = k;
+        for (int i = k + lane; i < nd; i += kWave) {
+            const double v = fabs(A[i * nd + k]);
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (lane == 0) perm[k] = bi;
+        if (bi != k)
+            for (int j = lane; j < nd; j += kWave) {
+                const double t = A[k * nd + j];
+                A[k * nd + j] = A[bi * nd + j];
+                A[bi * nd + j] = t;
+            }
+        __syncthreads();
+        const double piv = A[k * nd + k];
+        const double ip = 1.0 / piv;
+        __syncthreads();
+        for (int j = lane; j < nd; j += kWave) A[k * nd + j] = (j == k) ? ip : A[k * nd + j] * ip;
+        // column k of the other rows, read before any of them changes
+        for (int i = lane; i < nd; i += kWave) col[i] = A[i * nd + k];
+        __syncthreads();
+        for (int e = lane; e < nd * nd; e += kWave) {
+            const int i = e / nd, j = e - (e / nd) * nd;
+            if (i == k) continue;
+            const double f = col[i];
+            if (j == k)
+                A[e] = -f * ip;
+            else
+                A[e] -= f * A[k * nd + j];
+        }
+        __syncthreads();
+    }
+    // row interchanges of the factorisation become column interchanges of the inverse
+    for (int k = nd - 1; k >= 0; k--) {
+        const int q = perm[k];
+        if (q != k)
+            for (int i = lane; i < nd; i += kWave) {
+                const double t = A[i * nd + k];
+                A[i * nd + k] = A[i * nd + q];
+                A[i * nd + q] = t;
+            }
+        __syncthreads();
+    }
+    IT* out = inv + iptr[p];
+    for (int e = lane; e < nd * nd; e += kWave) {
+        const int j = e / nd, i = e - (e / nd) * nd;   // column-major: out[j nd + i] = Ainv[i][j]
+        out[e] = (IT)A[i * nd + j];
+    }
+}
+
+// y_p = A_p^{-1} r_p, one wave per patch, four patches per workgroup
+template <typename IT>
+__global__ __launch_bounds__(256) void k_patch_apply(int64_t P, const int32_t* pptr, const int32_t* pnodes,
+                                                     const int64_t* iptr, const IT* inv, const double* r,
+                                                     double* y, int nd_max) {
+    extern __shared__ double lds[];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t p = (int64_t)blockIdx.x * 4 + w;
+    double* rp = lds + (size_t)w * nd_max;
+    const bool valid = p < P;
+    const int p0 = valid ? pptr[p] : 0;
+    const int nd = valid ? 3 * (pptr[p + 1] - p0) : 0;
+    for (int l = lane; l < nd; l += kWave) rp[l] = r[3 * (int64_t)pnodes[p0 + l / 3] + l % 3];
+    __syncthreads();
+    if (!valid) return;
+    const IT* a = inv + iptr[p];
+    double* yo = y + 3 * (int64_t)p0;
+    for (int i = lane; i < nd; i += kWave) {
+        double acc = 0.0;
+        for (int j = 0; j < nd; j++) acc += (double)a[(int64_t)j * nd + i] * rp[j];
+        yo[i] = acc;
+    }
+}
+
+// z[3n + c] = sum over the patches holding node n of their y rows, in patch order
+__global__ __launch_bounds__(256) void k_patch_gather(int64_t N, const int32_t* cptr, const int32_t* cpos,
+                                                      const double* y, double* z) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = cptr[n]; k < cptr[n + 1]; k++) {
+        const double* q = y + cpos[k];
+        s0 += q[0];
+        s1 += q[1];
+        s2 += q[2];
+    }
+    z[3 * n] = s0;
+    z[3 * n + 1] = s1;
+    z[3 * n + 2] = s2;
+}
+
+}  // namespace
+
+int build_vertex_patches(cmg_ctx* c, const int32_t* elements) {
+    c->schwarz = false;
+    if (c->npe != 10) return CMG_SUCCESS;   // T4: every node is a vertex, patches would be block Jacobi
+    const int64_t N = c->N, E = c->E;
+    // patch of each vertex: itself and the midside nodes of its edges
+    std::vector<int32_t> vid(N, -1);
+    for (int64_t e = 0; e < E; e++)
+        for (int a = 0; a < 4; a++) vid[elements[e * 10 + a]] = 0;
+    int64_t P = 0;
+    for (int64_t n = 0; n < N; n++)
+        if (vid[n] == 0) vid[n] = (int32_t)P++;
+    std::vector<std::vector<int32_t>> patch(P);
+    for (int64_t n = 0; n < N; n++)
+        if (vid[n] >= 0) patch[vid[n]].push_back((int32_t)n);
+    static const int kEdge[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};   // T10 midside order
+    for (int64_t e = 0; e < E; e++)
+        for (int k = 0; k < 6; k++) {
+            const int32_t m = elements[e * 10 + 4 + k];
+            patch[vid[elements[e * 10 + kEdge[k][0]]]].push_back(m);
+            patch[vid[elements[e * 10 + kEdge[k][1]]]].push_back(m);
+        }
+    int np_max = 0;
+    for (auto& q : patch) {
+        std::sort(q.begin() + 1, q.end());
+        q.erase(std::unique(q.begin() + 1, q.end()), q.end());
+        np_max = std::max(np_max, (int)q.size());
+    }
+    if (3 * np_max > kMaxPatchDofs) return CMG_SUCCESS;   // stays with block Jacobi
+    std::vector<int32_t> pptr(P + 1, 0), pnodes;
+    std::vector<int64_t> sptr(P + 1, 0), iptr(P + 1, 0);
+    std::vector<int32_t> slots;
+    for (int64_t p = 0; p < P; p++) {
+        const auto& q = patch[p];
+        const int np = (int)q.size();
+        pptr[p + 1] = pptr[p] + np;
+        pnodes.insert(pnodes.end(), q.begin(), q.end());
+        for (int i = 0; i < np; i++)
+            for (int j = 0; j < np; j++) {
+                const int32_t* b = c->h_colind.data() + c->h_rowptr[q[i]];
+                const int32_t* e = c->h_colind.data() + c->h_rowptr[q[i] + 1];
+                const int32_t* f = std::lower_bound(b, e, q[j]);
+                slots.push_back((f != e && *f == q[j]) ? (int32_t)(c->h_rowptr[q[i]] + (f - b)) : -1);
+            }
+        sptr[p + 1] = (int64_t)slots.size();
+        iptr[p + 1] = iptr[p] + (int64_t)9 * np * np;
+    }
+    // node -> positions of its rows in y (y rows follow pnodes), in patch order
+    std::vector<int32_t> cnt(N + 1, 0);
+    for (int64_t k = 0; k < (int64_t)pnodes.size(); k++) cnt[pnodes[k] + 1]++;
+    for (int64_t n = 0; n < N; n++) cnt[n + 1] += cnt[n];
+    std::vector<int32_t> cpos(pnodes.size()), fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t k = 0; k < (int64_t)pnodes.size(); k++) cpos[fill[pnodes[k]]++] = (int32_t)(3 * k);
+    for (int64_t n = 0; n < N; n++)
+        if (cnt[n + 1] == cnt[n]) return CMG_SUCCESS;   // a node outside every patch: keep block Jacobi
+    hipStream_t s = c->stream;
+    CMG_TRY(upload(c->sw_pptr, pptr.data(), pptr.size(), s));
+    CMG_TRY(upload(c->sw_pnodes, pnodes.data(), pnodes.size(), s));
+    CMG_TRY(upload(c->sw_sptr, sptr.data(), sptr.size(), s));
+    CMG_TRY(upload(c->sw_slots, slots.data(), slots.size(), s));
+    CMG_TRY(upload(c->sw_iptr, iptr.data(), iptr.size(), s));
+    CMG_TRY(upload(c->sw_cptr, cnt.data(), cnt.size(), s));
+    CMG_TRY(upload(c->sw_cpos, cpos.data(), cpos.size(), s));
+    CMG_TRY(alloc(c->sw_inv, (size_t)iptr[P] * (c->fp32 ? sizeof(float) : sizeof(double))));
+    CMG_TRY(alloc(c->sw_y, (size_t)3 * pnodes.size() * sizeof(double)));
+    CMG_HIP_CHECK(hipStreamSynchronize(s));
+    c->sw_P = P;
+    c->sw_nd_max = 3 * np_max;
+    c->schwarz = true;
+    return CMG_SUCCESS;
+}
+
+void launch_patch_setup(cmg_ctx* c, const void* vals) {
+    const int ndm = c->sw_nd_max;
+    const size_t lds = (size_t)(ndm * ndm + ndm) * sizeof(double) + (size_t)ndm * sizeof(int);
+#define CMG_PSETUP(VT, IT)                                                                                       \
+    hipLaunchKernelGGL((k_patch_setup<VT, IT>), dim3((unsigned)c->sw_P), dim3(kWave), lds, c->stream, c->sw_P,  \
+                       c->sw_pptr.as<int32_t>(), c->sw_pnodes.as<int32_t>(), c->sw_sptr.as<int64_t>(),           \
+                       c->sw_slots.as<int32_t>(), c->sw_iptr.as<int64_t>(), (const VT*)vals, c->sw_inv.as<IT>(), ndm)
+    if (c->fp32)
+        CMG_PSETUP(float, float);
+    else
+        CMG_PSETUP(double, double);
+#undef CMG_PSETUP
+}
+
+void launch_patch_apply(cmg_ctx* c, const double* r, double* z) {
+    const int ndm = c->sw_nd_max;
+    const dim3 grid((unsigned)((c->sw_P + 3) / 4));
+    const size_t lds = (size_t)4 * ndm * sizeof(double);
+    if (c->fp32)
+        hipLaunchKernelGGL(k_patch_apply<float>, grid, dim3(256), lds, c->stream, c->sw_P, c->sw_pptr.as<int32_t>(),
+                           c->sw_pnodes.as<int32_t>(), c->sw_iptr.as<int64_t>(), c->sw_inv.as<float>(), r,
+                           c->sw_y.as<double>(), ndm);
+    else
+        hipLaunchKernelGGL(k_patch_apply<double>, grid, dim3(256), lds, c->stream, c->sw_P, c->sw_pptr.as<int32_t>(),
+                           c->sw_pnodes.as<int32_t>(), c->sw_iptr.as<int64_t>(), c->sw_inv.as<double>(), r,
+                           c->sw_y.as<double>(), ndm);
+    hipLaunchKernelGGL(k_patch_gather, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, c->N,
+                       c->sw_cptr.as<int32_t>(), c->sw_cpos.as<int32_t>(), c->sw_y.as<double>(), z);
+}
+
+}  // namespace cmg

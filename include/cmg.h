@@ -141,7 +141,14 @@ typedef struct cmg_options {
                                    transpose_tangent says, rebuilt whenever that configuration
                                    changes (CBSolverNewmarkBeta.cpp:384-390, 622-626, 657-702,
                                    857-866); off when both are 0 */
-    double reserved[2];
+    int32_t preconditioner;     /* of the Krylov solves: 1 3x3 nodal block Jacobi, 2 vertex-patch
+                                   additive Schwarz (T10: each element vertex with the midside
+                                   nodes of its edges, patch matrices inverted on the device;
+                                   block Jacobi for T4 and for krylov_method 1); 0 (default):
+                                   2 where it applies */
+    int32_t reserved_i;
+    double krylov_eta_max;      /* upper bound of the Eisenstat-Walker forcing term of the Newton
+                                   linear solves (default 0.5; 0 reads as the default) */
 } cmg_options;
 
 typedef struct cmg_solve_stats {

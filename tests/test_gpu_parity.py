@@ -189,12 +189,16 @@ def test_mass_matrix():
     assert np.abs(Mg - Mo).max() <= 1e-14 * np.abs(Mo).max()
 
 
+@pytest.mark.parametrize("pc", [1, 2], ids=["block_jacobi", "schwarz"])
 @pytest.mark.parametrize("method", [0, 1, 2], ids=["gmres", "bicgstab_graph", "bicgstab"])
-def test_spmv_and_krylov_solve(method):
-    """The Newmark system matrix K + M/(beta dt^2) at the unloaded state, dt = 1 ms."""
+def test_spmv_and_krylov_solve(method, pc):
+    """The Newmark system matrix K + M/(beta dt^2) at the unloaded state, dt = 1 ms;
+    block Jacobi or vertex-patch Schwarz preconditioning (the device-scalar BiCGStab
+    always uses block Jacobi)."""
     mesh, mats = lv_small()
     rng = np.random.default_rng(3)
-    with MechanicsContext(mesh, mats, krylov_restart=60, krylov_maxit=20000, krylov_method=method) as ctx:
+    with MechanicsContext(mesh, mats, krylov_restart=60, krylov_maxit=20000, krylov_method=method,
+                          preconditioner=pc) as ctx:
         Kt = ctx.bsr(ctx.tangent(mesh.coords.ravel(), 0.0))
         Mm = ctx.bsr(ctx.mass_matrix())
         vals = np.ascontiguousarray((Kt.data + Mm.data / 0.25e-6).reshape(-1))   # same block pattern
@@ -209,7 +213,8 @@ def test_spmv_and_krylov_solve(method):
         xs, st = ctx.linear_solve(vals, b, rtol=1e-12)
     import scipy.sparse.linalg as spla
     xr = spla.spsolve(K.tocsc(), b)
-    print(f"method {method} iterations {st.iterations}, rel err {np.linalg.norm(xs - xr) / np.linalg.norm(xr):.2e}")
+    print(f"method {method} preconditioner {pc} iterations {st.iterations}, "
+          f"rel err {np.linalg.norm(xs - xr) / np.linalg.norm(xr):.2e}")
     assert st.converged
     assert np.linalg.norm(xs - xr) <= 1e-7 * np.linalg.norm(xr)
 
@@ -275,15 +280,16 @@ def test_static_mixed_precision_converges_to_fp64_root():
         assert err <= 1e-8
 
 
-@pytest.mark.parametrize("method", [0, 1, 2], ids=["gmres", "bicgstab_graph", "bicgstab"])
-def test_newmark_steps_match_oracle(method):
+@pytest.mark.parametrize("method,pc", [(0, 1), (1, 1), (2, 1), (0, 2), (2, 2)],
+                         ids=["gmres", "bicgstab_graph", "bicgstab", "gmres_schwarz", "bicgstab_schwarz"])
+def test_newmark_steps_match_oracle(method, pc):
     """Newmark (beta=1/4, gamma=1/2) with Holzapfel + active tension + cavity pressure."""
     mesh, mats = lv_small()
     cpu = CpuMechanics(mesh, mats)
     plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
     dt = 0.005
     with MechanicsContext(mesh, mats, newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15,
-                          krylov_method=method) as ctx:
+                          krylov_method=method, preconditioner=pc) as ctx:
         for k in range(1, 9):
             t = k * dt
             p = plug.apply(t, dt)[0]
@@ -400,17 +406,28 @@ def test_newton_reports_a_krylov_solve_without_update_as_failure():
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 def test_lv_beat_passes_activation_onset(precision):
     """The 20k-element LV beat through activation onset (0-40 ms, Newmark 1/4, 1/2,
-    dt = 1 ms with CBTiming halving): every step is accepted with its residual below
-    Solver.Precision.  On the first LV generator's apex needles Newton failed here
-    until dt reached its minimum, and a mixed-precision run once took a broken-down
-    Krylov solve for convergence."""
+    dt = 1 ms with CBTiming halving): every step is accepted, on the absolute test
+    (||r|| < Solver.Precision = 1e-8 N), on the relative one (||r|| < 1e-8 ||r_0||), or
+    on the step-norm test with ||r|| below 1e-6 N (such steps are counted and printed).
+    On the first LV generator's apex needles Newton failed here until dt reached its
+    minimum, and a mixed-precision run once took a broken-down Krylov solve for
+    convergence."""
     mesh = M.lv_mesh(20_000)
     recs = []
     with MechanicsContext(mesh, [MT.lv_holzapfel()], precision=precision) as ctx:
         SV.run(ctx, "newmark", SV.Timing(0.0, 0.04, 1e-3), SV.sinus_pulse(-16000.0, 0.0, 0.8),
-               on_step=lambda rec, st: recs.append((rec.time, rec.dt, st.residual_norm, st.converged_reason)))
+               on_step=lambda rec, st: recs.append((rec.time, rec.dt, st.residual_norm, st.converged_reason,
+                                                    st.initial_residual_norm)))
     halved = sum(1 for r in recs if r[1] < 0.999e-3)
-    print(f"{len(recs)} accepted steps, {halved} shorter than 1 ms")
+    by_norm = [r for r in recs if r[3] == 4]
+    print(f"{len(recs)} accepted steps, {halved} shorter than 1 ms, {len(by_norm)} on the step-norm test "
+          f"(residuals {[f'{r[2]:.1e}' for r in by_norm]})")
     assert recs[-1][0] >= 0.04 - 1e-9
-    for t, dt, rn, reason in recs:
-        assert reason > 0 and (rn < 1e-8 or reason in (3, 4)), (t, dt, rn, reason)
+    for t, dt, rn, reason, r0 in recs:
+        assert reason in (2, 3, 4), (t, dt, rn, reason)
+        if reason == 2:
+            assert rn < 1e-8, (t, dt, rn, reason)
+        elif reason == 3:
+            assert rn < 1e-8 * r0, (t, dt, rn, reason, r0)
+        else:
+            assert rn < 1e-6, (t, dt, rn, reason)
