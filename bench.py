@@ -25,6 +25,8 @@ import numpy as np  # noqa: E402
 # f 240 + K_e 7200 (FP64) / 3600 (FP32 values)
 BYTES_PER_ELEM = {0: 1548 + 240 + 7200, 1: 1548 + 240 + 3600}
 FLOPS_PER_ELEM = 42_700
+# residual only: inputs 1548 B + f 240 B
+RES_BYTES_PER_ELEM = 1548 + 240
 # measured HBM bytes per element of the default kernels (FP64 variant 6, mixed variant 5):
 # rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, calibrated on a copy kernel of known size,
 # on the 200,340-element LV (profiles/element_residual_tangent_lv200k_fp64.md)
@@ -43,18 +45,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--elements", type=int, default=200_000)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--beat-steps", type=int, default=2, help="Newmark steps of the beat to time (0: skip)")
-    ap.add_argument("--krylov", default="bicgstab,bicgstab_graph",
-                    help="Krylov methods to time the beat steps with (comma list of bicgstab, bicgstab_graph, gmres)")
-    ap.add_argument("--beat-mixed", type=int, default=1,
-                    help="also time the beat steps in mixed precision (BiCGStab) and report their displacement "
-                         "difference to the FP64 run (0: off)")
+    ap.add_argument("--beat-ms", type=float, default=10.0,
+                    help="measured window [0, this] ms of the LV beat (Newmark beta 1/4, gamma 1/2, dt 1 ms with "
+                         "CBTiming halving), in FP64 and mixed precision, with and without the learned warm start "
+                         "(0: skip)")
+    ap.add_argument("--beat-tol", default="1e-12,1e-12,1e-14",
+                    help="Newton abstol, rtol, stol of the beat window runs")
+    ap.add_argument("--beat-runs", default="fp64_bj,fp64,mixed,mixed_warm",
+                    help="comma list of fp64_bj (block Jacobi), fp64, mixed, mixed_warm (vertex-patch Schwarz)")
+    ap.add_argument("--beat-train", type=int, default=2,
+                    help="training windows for the warm start (Tmax +-15 %%, activation +-10 ms, seeded)")
+    ap.add_argument("--beat-cpu-steps", type=int, default=1,
+                    help="Newmark steps of the same beat run by the CPU reference solver for the beat's CPU "
+                         "baseline (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="CPU baseline sample: repeat 20k-element chunks of the mesh for about this long")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-element", type=int, default=0,
                     help="profiling mode: N launches of the element kernel (default variant) and of a "
                          "known-size copy kernel, nothing else")
+    ap.add_argument("--profile-assembly", type=int, default=0,
+                    help="profiling mode: N passes of element kernel + deterministic assembly "
+                         "(cmg_residual_tangent_device) and N launches of a known-size copy kernel, nothing else")
+    ap.add_argument("--profile-beat", type=int, default=0,
+                    help="profiling mode: the first N Newmark steps (1 ms) of the LV beat with the default solver")
     ap.add_argument("--sweep", action="store_true",
                     help="run the batched passive-parameter sweep (Guccione sets, quasi-static LV inflation) "
                          "instead of the element-kernel benchmark")
@@ -111,21 +125,56 @@ def main():
     t_eval = 0.12
     xd = None
 
-    def time_kernel(ctx, steps, warmup):
+    def time_kernel(ctx, steps, warmup, tangent=1):
         stream = ctx.L.cmg_stream(ctx.h)
         for _ in range(warmup):
-            ctx._check(ctx.L.cmg_element_kernel_device(ctx.h, xd.ptr, t_eval, 1), "element kernel")
+            ctx._check(ctx.L.cmg_element_kernel_device(ctx.h, xd.ptr, t_eval, tangent), "element kernel")
         ctx._check(ctx.L.cmg_synchronize(ctx.h), "sync")
         e0, e1 = Event(), Event()
         w0 = time.perf_counter()
         e0.record(stream)
         for _ in range(steps):
-            ctx._check(ctx.L.cmg_element_kernel_device(ctx.h, xd.ptr, t_eval, 1), "element kernel")
+            ctx._check(ctx.L.cmg_element_kernel_device(ctx.h, xd.ptr, t_eval, tangent), "element kernel")
         e1.record(stream)
         e1.synchronize()
         ctx._check(ctx.L.cmg_synchronize(ctx.h), "sync")
         return e0.elapsed_ms(e1) / steps, (time.perf_counter() - w0) / steps * 1e3
 
+    if args.profile_assembly > 0:
+        ctx = MechanicsContext(mesh, mats, precision=prec)
+        xd = DeviceArray(x.ravel())
+        fd_ = DeviceArray(np.zeros(ctx.n_dofs))
+        vals = DeviceArray(np.zeros(ctx.nnzb * 9, np.float32 if prec else np.float64))
+        ncopy = 64 * 1024 * 1024
+        src = DeviceArray(np.ones(ncopy))
+        dst = DeviceArray(np.zeros(ncopy))
+        a0 = time.perf_counter()
+        for _ in range(args.profile_assembly):
+            ctx._check(ctx.L.cmg_residual_tangent_device(ctx.h, xd.ptr, t_eval, 1, fd_.ptr, vals.ptr), "rt")
+        ctx._check(ctx.L.cmg_synchronize(ctx.h), "sync")
+        pass_ms = (time.perf_counter() - a0) / args.profile_assembly * 1e3
+        for _ in range(args.profile_assembly):
+            ctx._check(ctx.L.cmg_copy_device(ctx.h, src.ptr, dst.ptr, ncopy), "copy")
+        ctx._check(ctx.L.cmg_synchronize(ctx.h), "sync")
+        print(json.dumps({"profile": "residual + tangent + assembly + copy", "elements": E, "dofs": ctx.n_dofs,
+                          "blocks": ctx.nnzb, "pass_ms_wall": pass_ms, "copy_bytes_read": 8 * ncopy,
+                          "copy_bytes_written": 8 * ncopy}))
+        return
+    if args.profile_beat > 0:
+        ctx = MechanicsContext(mesh, mats, precision=prec)
+        plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
+        steps = []
+        for k in range(1, args.profile_beat + 1):
+            tk = k * 1e-3
+            plug.apply(tk, 1e-3)
+            ctx.set_pressures(plug.values(ctx.n_groups))
+            rc, st = ctx.newmark_step(tk, 1e-3)
+            steps.append({"t_ms": k, "rc": rc, "newton": st.newton_iterations, "krylov": st.krylov_iterations,
+                          "wall_s": st.wall_seconds, "tangent_s": st.tangent_seconds, "krylov_s": st.krylov_seconds})
+            log(f"[bench] profile step {k}: {steps[-1]}")
+        print(json.dumps({"profile": "Newmark steps", "elements": E, "dofs": ctx.n_dofs, "steps": steps}))
+        ctx.close()
+        return
     if args.profile_element > 0:
         ctx = MechanicsContext(mesh, mats, precision=prec)   # the default variant of the precision
         xd = DeviceArray(x.ravel())
@@ -178,51 +227,22 @@ def main():
     traffic = TRAFFIC_PER_ELEM[prec] * E if TRAFFIC_PER_ELEM[prec] else None
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch of this kernel on this mesh, "
+                                  "calibrated on a copy kernel (profiles/); bytes per element times E, not "
+                                  "re-measured in this run",
                 "bytes_per_element": BYTES_PER_ELEM[prec], "kernel": f"k_residual_tangent variant {best}",
                 "fp64_flops_per_s": FLOPS_PER_ELEM * E / (kernel_ms * 1e-3)}
+    # residual only (the line-search / convergence-check evaluation): 1,788 B and 3,160 flops per element
+    res_ms, _ = time_kernel(ctx, args.steps, args.warmup, tangent=0)
+    residual_only = {"elem_per_s": E / (res_ms * 1e-3), "ms": res_ms, "bytes_per_element": RES_BYTES_PER_ELEM,
+                     "achieved_GBps": RES_BYTES_PER_ELEM * E / (res_ms * 1e-3) / 1e9,
+                     "frac": RES_BYTES_PER_ELEM * E / (res_ms * 1e-3) / HBM_PEAK}
+    log(f"[bench] residual-only kernel: {res_ms:.3f} ms/launch, {E / (res_ms * 1e-3):.3e} elem/s, "
+        f"{residual_only['frac']:.2f} of 8 TB/s")
 
     beat = None
-    if args.beat_steps > 0:
-        beat = {}
-        runs = [(name.strip(), 0) for name in args.krylov.split(",")]
-        if args.beat_mixed:
-            runs.append(("bicgstab", 1))
-        final = {}
-        for name, bprec in runs:
-            method = {"gmres": 0, "bicgstab_graph": 1, "bicgstab": 2}[name]
-            # the element variant measured above for FP64; the precision's default otherwise
-            bctx = MechanicsContext(mesh, mats, precision=bprec, krylov_method=method,
-                                    **({"element_variant": best} if bprec == prec else {}))
-            plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
-            dt = 1e-3
-            times = []
-            its = []
-            key = name if bprec == 0 else name + "_mixed"
-            for k in range(1, args.beat_steps + 1):
-                tk = k * dt
-                plug.apply(tk, dt)
-                bctx.set_pressures(plug.values(bctx.n_groups))
-                b0 = time.perf_counter()
-                rc, st = bctx.newmark_step(tk, dt)
-                times.append(time.perf_counter() - b0)
-                its.append((st.newton_iterations, st.krylov_iterations))
-                log(f"[bench] {key} newmark step t={tk * 1e3:.0f} ms rc={rc} newton={st.newton_iterations} "
-                    f"krylov={st.krylov_iterations} {times[-1]:.3f} s")
-            ms, lay, lanes = bctx.spmv_tuning()
-            log(f"[bench] {key} SpMV layouts (ms): block {ms[0]:.3f}/{ms[1]:.3f}/{ms[2]:.3f}, "
-                f"chunk {ms[3]:.3f}/{ms[4]:.3f}/{ms[5]:.3f}, flat {ms[6]:.3f}/{ms[7]:.3f}/{ms[8]:.3f}; "
-                f"chosen layout {lay} x {lanes} lanes")
-            final[key] = bctx.get_state()["nodes"]
-            bctx.close()
-            beat[key] = {"steps": args.beat_steps, "dt": dt, "precision": "mixed" if bprec else "fp64",
-                         "s_per_step": float(np.mean(times)),
-                         "newton_krylov": its, "beat_800_steps_s_extrapolated": float(np.mean(times) * 800),
-                         "spmv_ms": [float(v) for v in ms], "spmv_layout": [lay, lanes]}
-        if "bicgstab_mixed" in final and "bicgstab" in final:
-            u = final["bicgstab"] - mesh.coords.ravel()
-            d = float(np.linalg.norm(final["bicgstab_mixed"] - final["bicgstab"]) / np.linalg.norm(u))
-            beat["bicgstab_mixed"]["rel_displacement_difference_vs_fp64"] = d
-            log(f"[bench] mixed vs FP64 displacement after {args.beat_steps} steps: {d:.2e} (relative)")
+    if args.beat_ms > 0:
+        beat = beat_window_runs(args, mesh, mats)
 
     cpu = None
     if not args.no_cpu:
@@ -254,12 +274,182 @@ def main():
            "data": "synthetic: generated truncated-ellipsoid LV, seeded smooth displacement (seed 1234)",
            "config": {"workload": "LV T10 residual+analytic tangent, Holzapfel-Ogden + active tension",
                       "elements": E, "nodes": mesh.n_nodes, "precision": args.precision,
-                      "element_variant": best, "variant_ms": variants, "assembly_ms_per_pass": asm_ms},
+                      "element_variant": best, "variant_ms": variants, "assembly_ms_per_pass": asm_ms,
+                      "residual_only": residual_only},
            "roofline": roofline, "cpu_baseline": cpu}
     if beat:
         out["beat"] = beat
+        if cpu is not None and beat.get("cpu_reference"):
+            cpu["beat"] = beat.pop("cpu_reference")
     print(json.dumps(out))
     ctx.close()
+
+
+def beat_window_runs(args, mesh, mats):
+    """Config 2's beat on the LV mesh over [0, --beat-ms]: Newmark beta 1/4, gamma 1/2, dt 1 ms
+    halved by CBTiming on failed steps (CBTiming.cpp:77-136), cavity pressure -16 kPa
+    sin^2 over 0.8 s, active tension 60 kPa sin^2 from the apex-to-base activation, all
+    runs at the same Newton tolerance.  Runs: FP64 with block Jacobi (the first GPU path),
+    FP64 and mixed precision with vertex-patch Schwarz, and mixed precision with the
+    learned warm start (trained on --beat-train jittered windows).  Each run's nodes are
+    compared with the FP64 run's at every time both accepted.  The CPU reference solver
+    (central-difference tangent, OpenMP) runs the first --beat-cpu-steps steps."""
+    from cardiomechanics_amd import materials as MT
+    from cardiomechanics_amd import mesh as M
+    from cardiomechanics_amd import solver as SV
+    from cardiomechanics_amd.context import MechanicsContext
+    from cardiomechanics_amd.warmstart import WarmStartNet
+
+    atol, rtol, stol = (float(x) for x in args.beat_tol.split(","))
+    T = args.beat_ms * 1e-3
+    dt0 = 1e-3
+    X0 = mesh.coords.ravel()
+    labels = [r.strip() for r in args.beat_runs.split(",") if r.strip()]
+    conf = {"fp64_bj": (0, 1), "fp64": (0, 2), "mixed": (1, 2), "mixed_warm": (1, 2)}
+    for lab in labels:
+        if lab not in conf:
+            raise SystemExit(f"--beat-runs: unknown run {lab}")
+
+    def window(prec, pc, mat, act=None, weights=None, record=False, keep=True):
+        msh = mesh if act is None else M.Mesh(coords=mesh.coords, elements=mesh.elements, bases=mesh.bases,
+                                              bc=mesh.bc, material=mesh.material, activation=act,
+                                              surfaces=mesh.surfaces, surface_id=mesh.surface_id)
+        plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
+        timing = SV.Timing(0.0, T, dt0)
+        res = dict(newton=0, krylov=0, accepted=0, halved=0, failed_attempts=0, kept=0, proposed=0, states={},
+                   samples=[])
+        with MechanicsContext(msh, [mat], precision=prec, preconditioner=pc, newton_atol=atol, newton_rtol=rtol,
+                              newton_stol=stol, newmark_beta=0.25, newmark_gamma=0.5) as ctx:
+            net = WarmStartNet(ctx, seed=5) if (weights is not None or record) else None
+            if weights is not None:
+                net.set_weights(weights)
+            t, p_prev, full = 0.0, 0.0, 0
+            t0 = time.perf_counter()
+            while t < T - 1e-12:
+                dt = timing.dt
+                plug.apply(t + dt, dt)
+                p = plug.values(ctx.n_groups)
+                ctx.set_pressures(p)
+                rate = (p[0] - p_prev) / dt
+                ok_dt = abs(dt - dt0) < 1e-15
+                if record and ok_dt and full >= 2:
+                    net.record_before_step(t + dt, rate)
+                if weights is not None and ok_dt and full >= 2:
+                    net.propose(t + dt, rate)
+                    res["proposed"] += 1
+                rc, st = ctx.newmark_step(t + dt, dt)
+                res["newton"] += st.newton_iterations
+                res["krylov"] += st.krylov_iterations
+                if rc == 2:
+                    t += dt
+                    p_prev = p[0]
+                    res["accepted"] += 1
+                    res["halved"] += not ok_dt
+                    res["kept"] += st.guess_source == 2
+                    full = full + 1 if ok_dt else 0
+                    if record and getattr(net, "_pending", None) is not None:
+                        net.record_after_step()
+                    if keep:
+                        res["states"][round(t * 1e7)] = ctx.get_state()["nodes"] - X0
+                else:
+                    res["failed_attempts"] += 1
+                    if record:
+                        net._pending = None
+                if not timing.compute_next(rc):
+                    raise RuntimeError(f"beat window: minimum time step at t={t + dt}")
+            res["wall"] = time.perf_counter() - t0
+            if record:
+                res["samples"] = net.samples
+            if net is not None:
+                net.close()
+        return res
+
+    runs = {}
+    nominal = MT.lv_holzapfel()
+    for lab in labels:
+        prec, pc = conf[lab]
+        weights = None
+        train = None
+        if lab == "mixed_warm":
+            rng = np.random.default_rng(0)
+            samples = []
+            t0 = time.perf_counter()
+            for b in range(args.beat_train):
+                m = MT.lv_holzapfel(tension_max=60e3 * rng.uniform(0.85, 1.15))
+                act = np.maximum(mesh.activation + rng.uniform(-0.01, 0.01), 0.0)
+                r = window(1, 2, m, act=act, record=True, keep=False)
+                samples += r["samples"]
+                log(f"[beat] training window {b}: {r['accepted']} steps, {len(r['samples'])} samples, "
+                    f"{r['wall']:.1f} s")
+            t_rec = time.perf_counter() - t0
+            with MechanicsContext(mesh, [nominal], precision=1) as tctx:
+                net = WarmStartNet(tctx, seed=5)
+                net.samples = samples
+                t1 = time.perf_counter()
+                hist = net.train(epochs=40, lr=2e-3)
+                weights = net.weights()
+                net.close()
+            train = {"windows": args.beat_train, "samples": len(samples), "record_s": t_rec,
+                     "train_s": time.perf_counter() - t1, "loss_first": hist[0], "loss_last": hist[-1]}
+        r = window(prec, pc, nominal, weights=weights)
+        r["train"] = train
+        runs[lab] = r
+        log(f"[beat] {lab}: {r['accepted']} steps ({r['halved']} shorter than 1 ms, {r['failed_attempts']} failed "
+            f"attempts), newton {r['newton']}, krylov {r['krylov']}, {r['wall']:.1f} s"
+            + (f", learned guess kept {r['kept']}/{r['proposed']}" if weights is not None else ""))
+    out = {"workload": f"LV beat window 0-{args.beat_ms:g} ms, Newmark beta 1/4 gamma 1/2, dt 1 ms with CBTiming "
+                       f"halving, -16 kPa sin^2 cavity pressure, 60 kPa sin^2 active tension apex->base",
+           "elements": mesh.n_elements, "newton_tolerance": [atol, rtol, stol], "runs": {}}
+    ref = runs.get("fp64")
+    umax = max(float(np.abs(u).max()) for u in ref["states"].values()) if ref else None
+    for lab, r in runs.items():
+        rec = {k: r[k] for k in ("wall", "accepted", "halved", "failed_attempts", "newton", "krylov")}
+        rec["precision"] = "mixed" if conf[lab][0] else "fp64"
+        rec["preconditioner"] = "block Jacobi" if conf[lab][1] == 1 else "vertex-patch Schwarz"
+        if r["train"]:
+            rec["warm_start"] = dict(r["train"], kept=r["kept"], proposed=r["proposed"])
+        if ref is not None and lab != "fp64":
+            common = sorted(set(ref["states"]) & set(r["states"]))
+            if common:
+                e_rel = [float(np.linalg.norm(r["states"][k] - ref["states"][k]) /
+                               max(np.linalg.norm(ref["states"][k]), 1e-300)) for k in common]
+                e_win = [float(np.abs(r["states"][k] - ref["states"][k]).max()) / umax for k in common]
+                rec["vs_fp64"] = {"common_times": len(common), "max_rel_displacement_difference": max(e_rel),
+                                  "last_rel_displacement_difference": e_rel[-1],
+                                  "max_abs_difference_over_window_max_displacement": max(e_win)}
+            if ref["wall"] > 0:
+                rec["speedup_vs_fp64"] = ref["wall"] / r["wall"]
+        out["runs"][lab] = rec
+    if umax is not None:
+        out["max_displacement_m"] = umax
+    if "fp64_bj" in runs and "mixed_warm" in runs:
+        out["speedup_mixed_warm_vs_fp64_bj"] = runs["fp64_bj"]["wall"] / runs["mixed_warm"]["wall"]
+    # the CPU reference solver on the first steps of the same beat
+    if not args.no_cpu and args.beat_cpu_steps > 0:
+        from oracle.drivers import CpuReferenceSolver
+        import oracle as O
+        ref_cpu = CpuReferenceSolver(mesh, [nominal])
+        plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
+        walls, stats = [], []
+        for k in range(1, args.beat_cpu_steps + 1):
+            tk = k * dt0
+            plug.apply(tk, dt0)
+            ref_cpu.set_pressures(plug.values(1))
+            rc, st = ref_cpu.newmark_step(tk, dt0, atol=atol, rtol=rtol, stol=stol)
+            walls.append(st["wall_s"])
+            stats.append(st)
+            log(f"[beat] CPU reference step t={tk * 1e3:.0f} ms: rc {rc}, newton {st['newton']:.0f}, krylov "
+                f"{st['krylov']:.0f}, {st['wall_s']:.1f} s (tangent {st['tangent_s']:.1f} s, linear "
+                f"{st['linear_s']:.1f} s)")
+        ref_cpu.close()
+        out["cpu_reference"] = {
+            "value": float(np.mean(walls)), "unit": "s per Newmark step", "cores": int(O.lib().orc_num_threads()),
+            "kind": "port", "steps": args.beat_cpu_steps,
+            "sample": f"the first {args.beat_cpu_steps} Newmark step(s) of the same beat (t = 1 ms ...), reference "
+                      f"algorithm: central-difference element tangents (h = Solver.Epsilon), BiCGStab + block "
+                      f"Jacobi in place of MUMPS LU, same Newton tolerance, OpenMP",
+            "newton_krylov": [[s_["newton"], s_["krylov"]] for s_ in stats]}
+    return out
 
 
 def beat_full(args):

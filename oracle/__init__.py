@@ -76,6 +76,16 @@ def lib():
             "orc_active_stress_model": (None, [_M, _D, ctypes.c_double, ctypes.c_double, _D]),
             "orc_t10_shape": (None, [_D, _D, _D]),
             "orc_shape_batch": (None, [ctypes.c_int32, ctypes.c_int64, _I32, _D, _D, _D]),
+            "orc_cpu_solver_create": (ctypes.c_void_p, [ctypes.c_int32, ctypes.c_int64, _I32, ctypes.c_int64, _D, _D,
+                                                        _D, _D, _U8, _M, ctypes.c_int32, _I32, _D, ctypes.c_int32,
+                                                        ctypes.c_int64, _I32, _I32, ctypes.c_int32]),
+            "orc_cpu_solver_destroy": (None, [ctypes.c_void_p]),
+            "orc_cpu_set_pressures": (None, [ctypes.c_void_p, _D]),
+            "orc_cpu_get_state": (None, [ctypes.c_void_p, _D, _D, _D]),
+            "orc_cpu_set_state": (None, [ctypes.c_void_p, _D, _D, _D]),
+            "orc_cpu_newmark_step": (ctypes.c_int32, [ctypes.c_void_p] + [ctypes.c_double] * 8 + [ctypes.c_int32, _D]),
+            "orc_cpu_static_step": (ctypes.c_int32, [ctypes.c_void_p] + [ctypes.c_double] * 6 + [ctypes.c_int32, _D]),
+            "orc_fd_epsilon": (ctypes.c_double, [_D, ctypes.c_int64, ctypes.c_double]),
             "orc_mass_batch": (None, [ctypes.c_int32, ctypes.c_int64, _I32, _D, _D, _U8, ctypes.c_int32, _D]),
             "orc_t10_residual": (ctypes.c_int32, [_D, _D, _D, ctypes.c_double, _U8, _M, ctypes.c_double,
                                                   ctypes.c_double, _D]),

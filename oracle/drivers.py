@@ -274,3 +274,88 @@ class CpuMechanics:
         for s in range(xs.shape[0]):
             vols[self.sgroup[s]] += O.surface_volume(xs[s])
         return vols
+
+
+class CpuReferenceSolver:
+    """The reference's solver steps with its central-difference element tangent on the
+    host cores (oracle/cpu_solver.cpp): orc_cpu_newmark_step restates
+    CBSolverNewmarkBeta::SolverStep (CBSolverNewmarkBeta.cpp:829-1019), orc_cpu_static_step
+    CBSolverEquilibrium::SolverStep (CBSolverEquilibrium.cpp:234-363); the tangent is the
+    FD Jacobian with h = CBSolver::CalcFiniteDifferencesEpsilon (CBSolver.cpp:1610-1625);
+    the linear solve is BiCGStab with block Jacobi in place of the reference's MUMPS LU.
+    The CPU baseline of the benchmark's beat and the FD-tangent solver path."""
+
+    STATS = ("newton", "krylov", "reason", "residual", "initial_residual", "tangent_s", "linear_s", "wall_s")
+
+    def __init__(self, mesh, materials, eps=1e-12):
+        import ctypes
+        self.mesh = mesh
+        self.ed = O.ElementData(mesh)
+        self.eps = eps
+        self.n = 3 * mesh.n_nodes
+        self._keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a
+        marr = O.materials_array(materials)
+        self._keep.append(marr)
+        coords = arr(mesh.coords, np.float64)
+        if mesh.surfaces.shape[0]:
+            sconn = arr(mesh.surfaces, np.int32)
+            gids = np.unique(mesh.surface_id)
+            sgroup = arr(np.searchsorted(gids, mesh.surface_id), np.int32)
+            nps, S, ng = mesh.surfaces.shape[1], mesh.surfaces.shape[0], len(gids)
+        else:
+            sconn = arr(np.zeros(1), np.int32)
+            sgroup = arr(np.zeros(1), np.int32)
+            nps, S, ng = 3, 0, 1
+        self.n_groups = ng
+        D, I32, U8 = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint8)
+        ed = self.ed
+        self.h = O.lib().orc_cpu_solver_create(
+            ed.npe, mesh.n_elements, ed.conn.ctypes.data_as(I32), mesh.n_nodes, coords.ctypes.data_as(D),
+            ed.dNdX.ctypes.data_as(D), ed.bases.ctypes.data_as(D), ed.w.ctypes.data_as(D),
+            ed.bcmask.ctypes.data_as(U8), marr, len(materials), ed.matidx.ctypes.data_as(I32),
+            ed.tact.ctypes.data_as(D), nps, S, sconn.ctypes.data_as(I32), sgroup.ctypes.data_as(I32), ng)
+
+    def close(self):
+        if getattr(self, "h", None):
+            O.lib().orc_cpu_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_pressures(self, p):
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(p, float), (self.n_groups,)))
+        O.lib().orc_cpu_set_pressures(self.h, p.ctypes.data_as(O._D))
+
+    def state(self):
+        out = [np.zeros(self.n) for _ in range(3)]
+        O.lib().orc_cpu_get_state(self.h, *[a.ctypes.data_as(O._D) for a in out])
+        return dict(zip(("nodes", "velocity", "acceleration"), out))
+
+    def set_state(self, nodes=None, velocity=None, acceleration=None):
+        args = []
+        for a in (nodes, velocity, acceleration):
+            if a is None:
+                args.append(None)
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                self._keep.append(a)
+                args.append(a.ctypes.data_as(O._D))
+        O.lib().orc_cpu_set_state(self.h, *args)
+
+    def newmark_step(self, time, dt, beta=0.25, gamma=0.5, atol=1e-8, rtol=1e-8, stol=1e-8, maxit=50):
+        st = np.zeros(8)
+        rc = O.lib().orc_cpu_newmark_step(self.h, time, dt, beta, gamma, self.eps, atol, rtol, stol, maxit,
+                                          st.ctypes.data_as(O._D))
+        return rc, dict(zip(self.STATS, st.tolist()))
+
+    def static_step(self, time, dt, atol=1e-8, rtol=1e-8, stol=1e-8, maxit=50):
+        st = np.zeros(8)
+        rc = O.lib().orc_cpu_static_step(self.h, time, dt, self.eps, atol, rtol, stol, maxit,
+                                         st.ctypes.data_as(O._D))
+        return rc, dict(zip(self.STATS, st.tolist()))

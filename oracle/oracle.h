@@ -138,6 +138,27 @@ int32_t orc_cs_tangent_batch(int32_t npe, int64_t n_elem, const int32_t* conn, c
                              const uint8_t* bcmask, const orc_material* mats, const int32_t* matidx,
                              const double* tact, double time, double* Ke, int32_t* status);
 int32_t orc_num_threads(void);
+void orc_shape_batch(int32_t npe, int64_t n_elem, const int32_t* conn, const double* coords, double* dNdX,
+                     double* detJ);
+void orc_mass_batch(int32_t npe, int64_t n_elem, const int32_t* conn, const double* w, const double* density,
+                    const uint8_t* bcmask, int32_t consistent, double* Me);
+
+/* ---- CPU solver steps with the reference's finite-difference tangent (cpu_solver.cpp) */
+typedef struct orc_cpu_solver orc_cpu_solver;
+orc_cpu_solver* orc_cpu_solver_create(int32_t npe, int64_t n_elem, const int32_t* conn, int64_t n_nodes,
+                                      const double* coords, const double* dNdX, const double* bases, const double* w,
+                                      const uint8_t* bcmask, const orc_material* mats, int32_t n_mats,
+                                      const int32_t* matidx, const double* tact, int32_t nps, int64_t n_surf,
+                                      const int32_t* sconn, const int32_t* sgroup, int32_t n_groups);
+void orc_cpu_solver_destroy(orc_cpu_solver* s);
+void orc_cpu_set_pressures(orc_cpu_solver* s, const double* p);
+void orc_cpu_get_state(const orc_cpu_solver* s, double* nodes, double* vel, double* acc);
+void orc_cpu_set_state(orc_cpu_solver* s, const double* nodes, const double* vel, const double* acc);
+int32_t orc_cpu_newmark_step(orc_cpu_solver* s, double time, double dt, double beta, double gamma, double eps,
+                             double atol, double rtol, double stol, int32_t maxit, double* stats);
+int32_t orc_cpu_static_step(orc_cpu_solver* s, double time, double dt, double eps, double atol, double rtol,
+                            double stol, int32_t maxit, double* stats);
+double orc_fd_epsilon(const double* a, int64_t n, double eps);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,12 @@ def ellipsoid_fixture():
     return mesh
 
 
+def lv_with_tension_fn(kind, **kw):
+    m = MT.lv_holzapfel()
+    m.tension_fn, m.fn = MT.function_params(kind, **kw)
+    return M.lv_mesh(dims=(2, 4, 8)), [m]
+
+
 CASES = {
     "cube_guccione": lambda: cube_small(2),
     "lv_holzapfel_active": lambda: lv_small(),
@@ -61,6 +67,8 @@ CASES = {
     "cube_t4_guccione": lambda: (M.cube_mesh(2, order=4), [MT.cube_guccione()]),
     "lv_holzapfel_tanh": lambda: (M.lv_mesh(dims=(2, 4, 8)), [MT.with_tanh(MT.lv_holzapfel(), PeakTension=60.0)]),
     "cube_t4_guccione_tanh": lambda: (M.cube_mesh(2, order=4), [MT.with_tanh(MT.cube_guccione())]),
+    "lv_holzapfel_sinusdriver": lambda: lv_with_tension_fn("SinusDriver", start=0.0, stop=0.3),
+    "lv_holzapfel_doublehill": lambda: lv_with_tension_fn("DoubleHill", start=0.0, stop=10.0, onset=0.02),
 }
 
 
@@ -180,13 +188,27 @@ def test_surface_forces_and_cavity_volume(surface_type):
     assert vg[0] < 0   # endocardial normals point into the cavity, as in the reference's ellipsoid.sur
 
 
-def test_mass_matrix():
+@pytest.mark.parametrize("consistent", [1, 0], ids=["consistent", "lumped"])
+def test_mass_matrix(consistent):
+    """consistent (CBElementSolidT10.cpp:352-394) or lumped (:396-422) mass"""
     mesh, mats = lv_small()
-    cpu = CpuMechanics(mesh, mats)
-    with MechanicsContext(mesh, mats) as ctx:
+    cpu = CpuMechanics(mesh, mats, consistent_mass=bool(consistent))
+    with MechanicsContext(mesh, mats, consistent_mass=consistent) as ctx:
         Mg = ctx.bsr(ctx.mass_matrix()).toarray()
     Mo = cpu.M.toarray()
+    if not consistent:
+        assert np.count_nonzero(Mo - np.diag(np.diag(Mo))) == 0
     assert np.abs(Mg - Mo).max() <= 1e-14 * np.abs(Mo).max()
+
+
+def test_t4_static_load_steps_match_oracle():
+    """Linear tetrahedra (T4, one quadrature point, CBElementSolidT4.cpp:130-210) in the
+    quasi-static pressure ramp on the cube."""
+    mesh, mats = M.cube_mesh(2, order=4), [MT.cube_guccione()]
+    res = run_static_pair(mesh, mats, 2000.0, 3, newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15)
+    for t, err, nit, kit in res:
+        print(f"T4 t={t:.2f} newton={nit} krylov={kit} rel err={err:.2e}")
+        assert err <= 1e-8
 
 
 @pytest.mark.parametrize("pc", [1, 2], ids=["block_jacobi", "schwarz"])

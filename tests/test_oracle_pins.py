@@ -67,6 +67,61 @@ def test_analytic_functions_closed_forms():
         assert MT.analytic(kind, fn, t) == pytest.approx(O.analytic(kind, fn, t), rel=1e-14)
 
 
+def test_sinus_driver_closed_form():
+    """SinusDriver (CBAnalyticFunction.h:105-119): sin(3.14159 (t - t0) / (t1 - t0)) on the
+    closed window [t0, t1], 0 outside.  The truncated pi leaves sin(3.14159) = 2.65e-6 at t1."""
+    fn = [0.1, 0.5] + [0] * 10
+    f = lambda t: O.analytic(O.FN_SINUSDRIVER, fn, t)     # noqa: E731
+    assert f(0.1) == 0.0 and f(0.0999) == 0.0 and f(0.5001) == 0.0
+    assert f(0.5) == pytest.approx(2.6535897933527304e-06, rel=1e-9)
+    assert f(0.3) == pytest.approx(0.99999999999912, abs=1e-14)        # the top: sin(3.14159 / 2)
+    for t in (0.15, 0.2):   # symmetric about the middle only to the truncation of pi
+        assert 1e-7 < abs(f(t) - f(0.6 - t)) < 3e-6
+    assert f(0.1 + 0.4 / 6) == pytest.approx(math.sin(3.14159 / 6), rel=1e-14)
+
+
+def test_double_hill_shape_at_its_hill_offsets():
+    """DoubleHill (CBAnalyticFunction.h:123-176, Stergiopulos 1996): at t - onset = the
+    contraction offset the contraction Hill term is exactly 1/2, at the relaxation offset
+    the relaxation term is exactly 1/2 -- swapping the offsets or the rate constants fails
+    these; before onset and outside the window it is 0, and it repeats with the period."""
+    c_off, r_off, a, b, onset, period = 0.215, 0.362, 1.32, 21.9, 0.05, 0.8
+    kind, fn = MT.function_params("DoubleHill", start=0.0, stop=10.0, period=period, contr_offset=c_off,
+                                  relax_offset=r_off, contr_rate=a, relax_rate=b, onset=onset)
+    mx = fn[11]
+    f = lambda t: O.analytic(kind, fn, t)     # noqa: E731
+    assert f(onset + c_off) == pytest.approx(0.5 / (1.0 + (c_off / r_off) ** b) / mx, rel=1e-13)
+    hr = (r_off / c_off) ** a
+    assert f(onset + r_off) == pytest.approx(hr / (1.0 + hr) * 0.5 / mx, rel=1e-13)
+    assert f(onset / 2) == 0.0 and f(onset) == 0.0
+    for t in (0.1, 0.3, 0.5):
+        assert f(t + period) == pytest.approx(f(t), rel=1e-12)
+    fn2 = list(fn)
+    fn2[1] = 0.6
+    assert O.analytic(kind, fn2, 0.61) == 0.0 and O.analytic(kind, fn2, 0.6) > 0.0
+
+
+def test_adam_restatement_matches_torch():
+    """The warm-start trainer's Adam step (oracle/mlp.py) against torch.optim.Adam on the
+    same gradients for five steps."""
+    torch = pytest.importorskip("torch")
+    from oracle import mlp as OM
+
+    rng = np.random.default_rng(3)
+    w = rng.normal(size=50)
+    grads = [rng.normal(size=50) for _ in range(5)]
+    tw = torch.tensor(w.copy(), dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([tw], lr=2e-3, betas=(0.9, 0.999), eps=1e-8)
+    m = np.zeros(50)
+    v = np.zeros(50)
+    for t, g in enumerate(grads, start=1):
+        opt.zero_grad()
+        tw.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+        w, m, v = OM.adam(w, g, m, v, t, 2e-3)
+        np.testing.assert_allclose(w, tw.detach().numpy(), rtol=0, atol=1e-15)
+
+
 # ---------------------------------------------------------------- laws
 @pytest.mark.parametrize("mat", [GUC, HO, HO0])
 def test_stress_free_reference(mat):
