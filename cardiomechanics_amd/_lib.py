@@ -26,6 +26,7 @@ EXPORTED = [
     "cmg_spmv", "cmg_spmv_layout", "cmg_linear_solve", "cmg_set_state", "cmg_get_state", "cmg_set_initial_guess",
     "cmg_static_step", "cmg_newmark_step", "cmg_newmark_energies", "cmg_newmark_solve", "cmg_trial_volumes", "cmg_commit_step",
     "cmg_copy_device", "cmg_residual_device_noalloc",
+    "cmg_set_land17_parameters", "cmg_get_land17_state", "cmg_set_land17_state",
     "cmg_batch_create", "cmg_batch_destroy", "cmg_batch_configure", "cmg_batch_info_get", "cmg_batch_static_step",
     "cmg_batch_get_nodes", "cmg_batch_set_nodes", "cmg_batch_residual", "cmg_batch_cavity_volumes",
     "cmg_mlp_num_weights", "cmg_mlp_create", "cmg_mlp_destroy", "cmg_mlp_get_weights", "cmg_mlp_set_weights",
@@ -137,6 +138,9 @@ def load():
         "cmg_set_state": (ctypes.c_int, [P, P, P, P]),
         "cmg_get_state": (ctypes.c_int, [P, P, P, P, P]),
         "cmg_set_initial_guess": (ctypes.c_int, [P, P]),
+        "cmg_set_land17_parameters": (ctypes.c_int, [P, i32, P]),
+        "cmg_get_land17_state": (ctypes.c_int, [P, P, P]),
+        "cmg_set_land17_state": (ctypes.c_int, [P, P, dbl]),
         "cmg_static_step": (ctypes.c_int, [P, dbl, dbl, P]),
         "cmg_newmark_step": (ctypes.c_int, [P, dbl, dbl, P]),
         "cmg_newmark_energies": (ctypes.c_int, [P, P, P]),

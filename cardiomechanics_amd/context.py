@@ -102,6 +102,10 @@ class MechanicsContext:
         self.rowptr = np.zeros(self.n_nodes + 1, np.int64)
         self.colind = np.zeros(self.nnzb, np.int32)
         self._check(self.L.cmg_pattern(self.h, _ptr(self.rowptr), _ptr(self.colind)), "cmg_pattern")
+        for i, m in enumerate(self.materials):
+            if getattr(m, "land17", None) is not None:
+                p = np.ascontiguousarray(m.land17, dtype=np.float64)
+                self._check(self.L.cmg_set_land17_parameters(self.h, i, _ptr(p)), "cmg_set_land17_parameters")
 
     # -- plumbing -------------------------------------------------------------
     def _check(self, rc, where):
@@ -244,6 +248,17 @@ class MechanicsContext:
         out = [np.zeros(self.n_dofs) for _ in range(4)]
         self._check(self.L.cmg_get_state(self.h, *[_ptr(a) for a in out]), "cmg_get_state")
         return dict(zip(("nodes", "velocity", "acceleration", "increment"), out))
+
+    def land17_state(self):
+        """Land17 state [8][E] (XS XW TRPN TmBlocked ZetaS ZetaW Cd lambda) and its commit time"""
+        st = np.zeros((8, self.n_elements))
+        t = ctypes.c_double()
+        self._check(self.L.cmg_get_land17_state(self.h, _ptr(st), ctypes.byref(t)), "cmg_get_land17_state")
+        return st, t.value
+
+    def set_land17_state(self, state, commit_time):
+        st = np.ascontiguousarray(state, dtype=np.float64).reshape(8, self.n_elements)
+        self._check(self.L.cmg_set_land17_state(self.h, _ptr(st), float(commit_time)), "cmg_set_land17_state")
 
     def set_initial_guess(self, increment):
         inc = np.ascontiguousarray(increment, dtype=np.float64).ravel()

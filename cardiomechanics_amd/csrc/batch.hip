@@ -437,6 +437,9 @@ int cmg_batch_create(cmg_ctx* c, int32_t n_sets, const cmg_material* materials, 
         // the batched solve is a conjugate gradient: it needs symmetric tangents, which the
         // Holzapfel law as coded does not give (its isochoric factor is 1 and its switch
         // enters the stress without its derivative, CBConstitutiveModelHolzapfel.cpp:89,121,148-154)
+        if (materials[i].tension_fn == CMG_FN_LAND17)
+            return fail("cmg_batch_create: the Land17 tension model (per-element history) is not supported in "
+                        "the parameter batch");
         if (materials[i].law != CMG_LAW_GUCCIONE)
             return fail("cmg_batch_create: the batched conjugate gradient needs symmetric tangents; only the "
                         "Guccione law (symmetric with FromFunction, TanH or no active tension) is accepted");

@@ -15,6 +15,7 @@
 
 #include "batch_kernels.h"
 #include "cmg_ctx.h"
+#include "land17_calcium.h"
 
 namespace cmg {
 namespace {
@@ -43,7 +44,7 @@ double now_s() {
 
 int to_dev_material(const cmg_material& m, DevMaterial& d) {
     if (m.law != CMG_LAW_GUCCIONE && m.law != CMG_LAW_HOLZAPFEL) return fail("unknown material law");
-    if (m.tension_fn < CMG_FN_NONE || m.tension_fn > CMG_FN_TANH) return fail("unknown tension function");
+    if (m.tension_fn < CMG_FN_NONE || m.tension_fn > CMG_FN_LAND17) return fail("unknown tension function");
     if (m.tension_fn == CMG_FN_TANH &&
         !(m.fn[0] > 0.0 && m.fn[4] >= 0.0 && m.fn[5] > 0.0 && m.fn[7] >= 0.0 && m.fn[8] > 0.0))
         return fail("TanH tension: cycleLength, ContractionTime and RelaxationTime must be positive, "
@@ -111,6 +112,38 @@ int64_t find_slot(const cmg_ctx* c, int64_t r, int32_t col) {
     const int32_t* p = std::lower_bound(b, e, col);
     if (p == e || *p != col) return -1;
     return c->h_rowptr[r] + (p - b);
+}
+
+// Land17 defaults (InitParamsFromXml, CBTensionModelLand17.cpp:428-500): model constants,
+// startTime, cycleLength, calcium transient (Coppini), rateDependancy (OFF), kff, and the
+// initial state XS XW TRPN TmBlocked ZetaS ZetaW Cd
+const double kLand17Defaults[kLand17Params] = {2.0,   0.1,   0.805, 1.0,    5.0,  0.35, 0.182, 0.012, 0.25, 0.5,
+                                               0.0085, 0.615, 2.23, 25.0,  2.3,  -2.4, 120.0, 2.1,   7.0,  200.0,
+                                               20.0,  1.0,   0.0,  1.0,    0.0,  0.0,  1.0,   0.0,   0.0,  0.0,
+                                               1.0,   0.0,   0.0,  0.0,    0.0,  0.0,  0.0,   0.0,   0.0,  0.0};
+
+// the committed state of every Land17 element from its material's initial values
+// (lambda = 1, the undeformed configuration)
+int land17_reset_state(cmg_ctx* c) {
+    std::vector<double> st((size_t)8 * c->E, 0.0);
+    for (int64_t e = 0; e < c->E; e++) {
+        const double* p = c->h_l17_params.data() + (size_t)kLand17Params * c->h_matidx[e];
+        for (int k = 0; k < 7; k++) st[(size_t)k * c->E + e] = p[27 + k];
+        st[(size_t)7 * c->E + e] = 1.0;
+    }
+    CMG_HIP_CHECK(hipMemcpyAsync(c->l17_state.p, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return CMG_SUCCESS;
+}
+
+// Land17 commit after an accepted step at `time` with the nodes in x_dev
+int land17_commit(cmg_ctx* c, const double* x_dev, double time) {
+    if (!c->l17) return CMG_SUCCESS;
+    launch_land17_commit(c->npe, elem_args(c, x_dev, time), c->l17_state.as<double>(), c->stream);
+    CMG_HIP_CHECK(hipGetLastError());
+    c->l17_t_commit = time;
+    return CMG_SUCCESS;
 }
 
 // CSR incidence list node -> (item * width + local) in item order
@@ -232,6 +265,18 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
     for (int i = 0; i < n_materials; i++) CMG_TRY(to_dev_material(materials[i], dm[i]));
     c->n_mats = n_materials;
     CMG_TRY(upload(c->mats, dm.data(), dm.size(), s));
+    c->h_matidx.assign(E, 0);
+    if (mesh->material_index) std::copy(mesh->material_index, mesh->material_index + E, c->h_matidx.begin());
+    for (int i = 0; i < n_materials; i++) c->l17 = c->l17 || materials[i].tension_fn == CMG_FN_LAND17;
+    if (c->l17) {
+        c->h_l17_params.assign((size_t)n_materials * kLand17Params, 0.0);
+        for (int i = 0; i < n_materials; i++)
+            std::copy(kLand17Defaults, kLand17Defaults + kLand17Params, c->h_l17_params.begin() + (size_t)i * kLand17Params);
+        CMG_TRY(upload(c->l17_params, c->h_l17_params.data(), c->h_l17_params.size(), s));
+        CMG_TRY(upload(c->l17_cai, &kLand17Cai[0][0], (size_t)2 * kLand17CaSamples, s));
+        CMG_TRY(alloc(c->l17_state, (size_t)8 * E * sizeof(double)));
+        CMG_TRY(land17_reset_state(c.get()));
+    }
     // -- per-element SoA data
     std::vector<int32_t> conn_soa((size_t)npe * E);
     for (int64_t e = 0; e < E; e++)
@@ -505,6 +550,12 @@ ElemArgs elem_args(cmg_ctx* c, const double* x_dev, double time) {
     a.fe = c->fe.as<double>();
     a.Ke = c->Ke.p;
     a.corrupt = c->corrupt.as<int>();
+    if (c->l17) {
+        a.l17_params = c->l17_params.as<double>();
+        a.l17_state = c->l17_state.as<double>();
+        a.l17_cai = c->l17_cai.as<double>();
+        a.l17_dt_ms = (time - c->l17_t_commit) * 1000.0;
+    }
     return a;
 }
 
@@ -1557,6 +1608,7 @@ int cmg_static_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) {
     }
     launch_axpby(n, 1.0, d, 1.0, c->nodes.as<double>(), s);   // :359
     c->damping_valid = false;
+    CMG_TRY(land17_commit(c, c->nodes.as<double>(), time));
     CMG_HIP_CHECK(hipStreamSynchronize(s));
     return CMG_SUCCESS;
 }
@@ -1662,6 +1714,7 @@ int cmg_commit_step(cmg_ctx* c) {
     }
     launch_axpby(n, 1.0, c->d.as<double>(), 1.0, c->nodes.as<double>(), s);
     c->damping_valid = false;
+    CMG_TRY(land17_commit(c, c->nodes.as<double>(), c->trial_time));
     // increment history (features of a learned warm start)
     launch_scale_copy(n, 1.0, c->inc1.as<double>(), c->inc2.as<double>(), s);
     launch_scale_copy(n, 1.0, c->d.as<double>(), c->inc1.as<double>(), s);
@@ -1681,6 +1734,46 @@ int cmg_newmark_step(cmg_ctx* c, double time, double dt, cmg_step_stats* stats) 
     int rc = cmg_newmark_solve(c, time, dt, 0, stats);
     if (rc != CMG_SUCCESS) return rc;
     return cmg_commit_step(c);
+}
+
+int cmg_set_land17_parameters(cmg_ctx* c, int32_t material, const double* params) {
+    if (!c || !params) return fail("cmg_set_land17_parameters: null argument");
+    if (!c->l17) return fail("cmg_set_land17_parameters: no material of this context uses the Land17 model");
+    if (material < 0 || material >= c->n_mats) return fail("cmg_set_land17_parameters: material out of range");
+    if (!(params[8] > 0.0 && params[8] < 1.0 && params[9] > 0.0 && params[9] < 1.0 && params[23] > 0.0 &&
+          params[19] > 0.0 && params[20] > 0.0))
+        return fail("cmg_set_land17_parameters: rs and rw must lie in (0, 1), cycleLength, eta_l and eta_s must "
+                    "be positive");
+    if (params[24] != 0.0 && params[24] != 1.0)
+        return fail("cmg_set_land17_parameters: calcium transient must be 0 (Coppini) or 1 (Brixius)");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    std::copy(params, params + kLand17Params, c->h_l17_params.begin() + (size_t)material * kLand17Params);
+    CMG_HIP_CHECK(hipMemcpyAsync(c->l17_params.p, c->h_l17_params.data(), c->h_l17_params.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+    return land17_reset_state(c);
+}
+
+int cmg_get_land17_state(cmg_ctx* c, double* state, double* commit_time) {
+    if (!c) return fail("null context");
+    if (!c->l17) return fail("cmg_get_land17_state: no material of this context uses the Land17 model");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    if (state)
+        CMG_HIP_CHECK(hipMemcpyAsync(state, c->l17_state.p, (size_t)8 * c->E * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (commit_time) *commit_time = c->l17_t_commit;
+    return CMG_SUCCESS;
+}
+
+int cmg_set_land17_state(cmg_ctx* c, const double* state, double commit_time) {
+    if (!c || !state) return fail("cmg_set_land17_state: null argument");
+    if (!c->l17) return fail("cmg_set_land17_state: no material of this context uses the Land17 model");
+    CMG_HIP_CHECK(hipSetDevice(c->device));
+    CMG_HIP_CHECK(hipMemcpyAsync(c->l17_state.p, state, (size_t)8 * c->E * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+    CMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->l17_t_commit = commit_time;
+    return CMG_SUCCESS;
 }
 
 }  // extern "C"

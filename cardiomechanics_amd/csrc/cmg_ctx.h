@@ -120,6 +120,13 @@ struct cmg_ctx {
     bool trial_pending = false;
     double trial_time = 0.0, trial_dt = 0.0;
     cmg::DevBuf override_guess;
+    // Land17 tension model: parameter blocks, committed state [8][E], calcium tables and the
+    // time of the last commit (the state's time)
+    bool l17 = false;
+    cmg::DevBuf l17_params, l17_state, l17_cai;
+    std::vector<double> h_l17_params;
+    std::vector<int32_t> h_matidx;
+    double l17_t_commit = 0.0;
     // vertex-patch additive Schwarz preconditioner (schwarz.hip): patch node lists, slot
     // maps into the block pattern, inverse offsets, node -> patch-row positions, the
     // inverses and the per-patch results

@@ -50,6 +50,12 @@ struct ElemArgs {
     int set_mats = 0;
     int64_t set_x = 0, set_fe = 0, set_Ke_bytes = 0;
     const int32_t* active = nullptr;
+    // Land17 (materials with tension_fn CMG_FN_LAND17): parameter blocks [n_mats][40],
+    // committed state [8][E], calcium transients [2][1001], (time - commit time) in ms
+    const double* l17_params = nullptr;
+    const double* l17_state = nullptr;
+    const double* l17_cai = nullptr;
+    double l17_dt_ms = 0.0;
 };
 
 struct SurfArgs {
@@ -79,6 +85,10 @@ void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s);
 void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipStream_t s, int variant);
 void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s);
 void launch_t10_shape(int64_t E, const int32_t* conn_aos, const double* X, double* dNdX_soa, double* w, hipStream_t s);
+// Land17 commit: each Land17 element advances its state [8][E] in place from the
+// evaluation at its last quadrature point at coordinates a.x and time a.time
+void launch_land17_commit(int npe, const ElemArgs& a, double* state, hipStream_t s);
+constexpr int kLand17Params = 40;
 
 // ---- launchers (assembly_kernels.hip) ----
 // vector gather: f[dof] = sum over incidences; BC components zeroed

@@ -83,6 +83,7 @@ __device__ double analytic(int type, const double* fn, double time) {
 __device__ __forceinline__ double tension_of(const DevMaterial& m, double time, double tact, double& ts) {
     ts = 0.0;
     if (m.tension_fn == CMG_FN_NONE) return 0.0;
+    if (m.tension_fn == CMG_FN_LAND17) return 1.0;   // evaluated per quadrature point from the element state
     if (m.tension_fn == CMG_FN_TANH) {
         const double* f = m.fn;
         double t = time;
@@ -92,6 +93,107 @@ __device__ __forceinline__ double tension_of(const DevMaterial& m, double time, 
         return m.tension_max * f[3] * (tr * tr);
     }
     return m.tension_max * analytic(m.tension_fn, m.fn, time - tact);
+}
+
+// ---------------------------------------------------------------------------
+// Land17 (CBTensionModelLand17.cpp): one explicit Euler step of the cross-bridge ODEs from
+// the element's committed state per evaluation (CalcActiveTension, :298-372), the tension
+// Tmax * max(0, Ta + F_d) and its derivative in the fibre stretch.
+// ---------------------------------------------------------------------------
+struct L17 {
+    const double* p;   // parameter block [40] of the element's material
+    double st[8];      // committed XS XW TRPN TmBlocked ZetaS ZetaW Cd lambda
+    double dt_ms;      // (time - commit time) in ms
+    double cai;        // Cai(time) of the element
+    double tmax;
+};
+
+// CaCoppini / CaBrixius (:55-245): 1 ms samples since activation (+ startTime), wrapped
+// into the cycle, index clipped to [0, 1000]
+__device__ __forceinline__ double land17_cai(const double* p, const double* tab, double time, double tact) {
+    double t = time - (tact + p[22]);
+    while (t > p[23]) t = t - p[23];
+    int index = (int)(t / 1e-3);
+    if (index < 0) index = 0;
+    index = min(1000, index);
+    return tab[(p[24] != 0.0 ? 1001 : 0) + index];
+}
+
+__device__ __forceinline__ void load_l17(const ElemArgs& a, int64_t e, int mi, const DevMaterial& m, L17& s) {
+    s.p = a.l17_params + kLand17Params * mi;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s.st[k] = a.l17_state[(int64_t)k * a.E + e];
+    s.dt_ms = a.l17_dt_ms;
+    s.cai = land17_cai(s.p, a.l17_cai, a.time, a.tact[e]);
+    s.tmax = m.tension_max;
+}
+
+// tension at stretch lam; dT/dlambda into *dT and the advanced state into next[8] when
+// not null (constants of the constructor, :262-272)
+__device__ inline double land17_tension(const L17& s, double lam, double* dT, double* next) {
+#pragma clang fp contract(off)
+    const double* p = s.p;
+    const double TRPNn = p[0], TRPNk = p[1], Ca50_ = p[2], ku = p[3], nTM = p[4], TRPN50 = p[5], kuw = p[6];
+    const double rs = p[8], rw = p[9], gammas = p[10], gammaw = p[11], phi = p[12], Aeff = p[13];
+    const double beta0 = p[14], beta1 = p[15], Tref = p[16], a = p[17], k = p[18], eta_l = p[19], eta_s = p[20];
+    const double kws = p[21] * p[7];
+    const double cw = phi * kuw * ((1.0 - rs) * (1.0 - rw)) / ((1.0 - rs) * rw);
+    const double cs = phi * kws * ((1.0 - rs) * rw) / rs;
+    const double kwu = kuw * (1.0 / rw - 1.0) - kws;
+    const double ksu = (kws * rw * (1.0 / rs - 1.0));
+    const double A = Aeff * rs / ((1.0 - rs) * rw + rs);
+    const double dt = s.dt_ms;
+    const double XS0 = s.st[0], XW0 = s.st[1], TRPN0 = s.st[2], TB0 = s.st[3], ZS0 = s.st[4], ZW0 = s.st[5];
+    const double Cd0 = s.st[6], lam0 = s.st[7];
+    const bool rate = !(dt == 0.0 || p[25] == 0.0);
+    const double dldt = rate ? (lam - lam0) / dt : 0.0;
+    const double ddldt = rate ? 1.0 / dt : 0.0;
+    const double lam_m = (lam < 1.2) ? lam : 1.2;
+    const double dlm = (lam < 1.2) ? 1.0 : 0.0;
+    const double lm87 = (lam_m < 0.87) ? lam_m : 0.87;
+    const double dlm87 = (lam_m < 0.87) ? dlm : 0.0;
+    const double hraw = 1.0 + beta0 * (lam_m + lm87 - 1.87);
+    const double h = (hraw < 0.0) ? 0.0 : hraw;
+    const double dh = (hraw < 0.0) ? 0.0 : beta0 * (dlm + dlm87);
+    const double XU = (1.0 - TB0) - XW0 - XS0;
+    const double xb_ws = kws * XW0, xb_uw = kuw * XU, xb_wu = kwu * XW0, xb_su = ksu * XS0;
+    const double zs_pos = (ZS0 > 0.0) ? ZS0 : 0.0;
+    const double zs_neg = (ZS0 < -1.0) ? (-ZS0 - 1.0) : 0.0;
+    const double zs = fmax(zs_pos, zs_neg);
+    const double xb_su_gamma = gammas * zs * XS0;
+    const double XS = XS0 + dt * (xb_ws - xb_su - xb_su_gamma);
+    const double gr_w = (ZW0 < 0.0) ? -ZW0 : ZW0;
+    const double xb_wu_gamma = gammaw * gr_w * XW0;
+    const double XW = XW0 + dt * (xb_uw - xb_wu - xb_ws - xb_wu_gamma);
+    const double ZS = ZS0 + dt * (A * dldt - cs * ZS0);
+    const double ZW = ZW0 + dt * (A * dldt - cw * ZW0);
+    const double Taraw = h * (Tref / rs) * ((ZS + 1.0) * XS + ZW * XW);
+    const double Ta = (Taraw < 0.0) ? 0.0 : Taraw;
+    const double C_s = (lam - 1.0) - Cd0;
+    const double F_d = a * k * C_s;
+    const double Traw = Ta + F_d;
+    const double Tension = (Traw < 0.0) ? 0.0 : Traw;
+    if (dT) {
+        const double dZ = dt * (A * ddldt);
+        const double dTa = (Taraw < 0.0) ? 0.0 : (Tref / rs) * (dh * ((ZS + 1.0) * XS + ZW * XW) + h * (dZ * XS + dZ * XW));
+        *dT = (Traw < 0.0) ? 0.0 : s.tmax * (dTa + a * k);
+    }
+    if (next) {
+        const double Ca50 = Ca50_ + beta1 * (lam_m - 1.0);
+        const double TRPN = TRPN0 + dt * (TRPNk * (pow(s.cai / Ca50, TRPNn) * (1.0 - TRPN0) - TRPN0));
+        const double ktm_block = ku * pow(TRPN50, nTM) * 0.5 / (0.5 - rs * 0.5 - (1.0 - rs) * rw * 0.5);
+        const double TRPN_NP = fmin(100.0, pow(TRPN, (-nTM / 2.0)));
+        const double eta = (C_s > 0.0) ? eta_l : eta_s;
+        next[0] = XS;
+        next[1] = XW;
+        next[2] = TRPN;
+        next[3] = TB0 + dt * (ktm_block * TRPN_NP * XU - ku * pow(TRPN, nTM / 2.0) * TB0);
+        next[4] = ZS;
+        next[5] = ZW;
+        next[6] = Cd0 + dt * (k * C_s / eta);
+        next[7] = lam;
+    }
+    return s.tmax * Tension;
 }
 
 // ---------------------------------------------------------------------------
@@ -160,7 +262,7 @@ struct Law {
 
 // returns false for det F <= 0 (CBStatus::CORRUPT_ELEMENT)
 __device__ __forceinline__ bool eval_law(const DevMaterial& m, const double* F, double Ta, double ts,
-                                         bool want_tangent, Law& L) {
+                                         bool want_tangent, Law& L, const L17* l17 = nullptr) {
 #pragma clang fp contract(off)
     const double J = det3(F);
     if (!m.ignore_corrupt && (J <= 0.0)) return false;
@@ -247,7 +349,17 @@ __device__ __forceinline__ bool eval_law(const DevMaterial& m, const double* F, 
     if (Ta != 0.0) {
         const double I4f = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
         const double rs = 1. / sqrt(I4f);
-        if (m.tension_fn == CMG_FN_TANH) {
+        if (m.tension_fn == CMG_FN_LAND17) {
+            // S_ff = kff T(lambda) / lambda with the element's cross-bridge state
+            if (l17) {
+                const double lam = sqrt(I4f);
+                double dT = 0.0;
+                const double T = land17_tension(*l17, lam, want_tangent ? &dT : nullptr, nullptr);
+                const double kff = l17->p[26];
+                L.S[0] += kff * T * rs;
+                if (want_tangent) L.d00 += kff * (dT * rs * rs - T * rs * rs * rs);
+            }
+        } else if (m.tension_fn == CMG_FN_TANH) {
             // T(lambda) = Ta phi tanh^2(t_s / tau_c), phi = tanh(ld (lambda - l0)),
             // tau_c = tau_c0 + ld_up (1 - phi), clipped at 0; S_ff = kff T / lambda
             const double* f = m.fn;
@@ -375,9 +487,47 @@ __device__ __forceinline__ bool batch_select(const SurfArgs& a0, SurfArgs& a) {
 }
 
 // ---------------------------------------------------------------------------
-// residual only: one thread per element, FP64
+// Land17 state commit of an accepted step: the element's state advances from the
+// evaluation at its last quadrature point (the reference's last CalcActiveTension call
+// of the step, SaveStateVariablesAsNeeded, CBTensionModelLand17.cpp:274-293)
 // ---------------------------------------------------------------------------
 template <int NPE, int NQP>
+__global__ __launch_bounds__(128) void k_land17_commit(const ElemArgs a, double* state) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const int mi = a.matidx ? a.matidx[e] : 0;
+    const DevMaterial& m = a.mats[mi];
+    if (m.tension_fn != CMG_FN_LAND17) return;
+    const int64_t E = a.E;
+    double x[3 * NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; k++) {
+        const int64_t n = a.conn[(int64_t)k * E + e];
+        x[3 * k] = a.x[3 * n];
+        x[3 * k + 1] = a.x[3 * n + 1];
+        x[3 * k + 2] = a.x[3 * n + 2];
+    }
+    constexpr int q = NQP - 1;
+    double g[NPE][3];
+    load_g<NPE>(a, e, q, g);
+    double B[9], Binv[9], Floc[9];
+#pragma unroll
+    for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
+    local_F<NPE>(x, g, B, Binv, Floc);
+    L17 s;
+    load_l17(a, e, mi, m, s);
+    double next[8];
+    land17_tension(s, sqrt(Floc[0] * Floc[0] + Floc[3] * Floc[3] + Floc[6] * Floc[6]), nullptr, next);
+#pragma unroll
+    for (int k = 0; k < 8; k++) state[(int64_t)k * E + e] = next[k];
+}
+
+// ---------------------------------------------------------------------------
+// residual only: one thread per element, FP64
+// ---------------------------------------------------------------------------
+// LAND17: instantiation for contexts with a Land17 material (its state and
+// parameters are read only there; the other instantiations carry none of it)
+template <int NPE, int NQP, bool LAND17 = false>
 __global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a0) {
 #pragma clang fp contract(off)
     ElemArgs a;
@@ -389,6 +539,12 @@ __global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a0) 
     const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
     double ts;
     const double Ta = tension_of(m, a.time, a.tact[e], ts);
+    L17 l17s;
+    const L17* l17 = nullptr;
+    if (LAND17 && m.tension_fn == CMG_FN_LAND17) {
+        load_l17(a, e, a.matidx ? a.matidx[e] : 0, m, l17s);
+        l17 = &l17s;
+    }
     double x[3 * NPE];
 #pragma unroll
     for (int k = 0; k < NPE; k++) {
@@ -408,7 +564,7 @@ __global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a0) 
         for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
         local_F<NPE>(x, g, B, Binv, Floc);
         Law L;
-        if (!eval_law(m, Floc, Ta, ts, false, L)) {
+        if (!eval_law(m, Floc, Ta, ts, false, L, l17)) {
             atomicMin(a.corrupt, (int)e);
             return;
         }
@@ -436,7 +592,7 @@ __global__ __launch_bounds__(kResidualBlock) void k_residual(const ElemArgs a0) 
 // ---------------------------------------------------------------------------
 // residual + tangent: 9 lanes per element (see file header)
 // ---------------------------------------------------------------------------
-template <int NPE, int NQP, typename TT>
+template <int NPE, int NQP, typename TT, bool LAND17 = false>
 __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
     ElemArgs a;
     if (!batch_select(a0, a)) return;
@@ -451,6 +607,12 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
     const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
     double ts;
     const double Ta = tension_of(m, a.time, a.tact[e], ts);
+    L17 l17s;
+    const L17* l17 = nullptr;
+    if (LAND17 && m.tension_fn == CMG_FN_LAND17) {
+        load_l17(a, e, a.matidx ? a.matidx[e] : 0, m, l17s);
+        l17 = &l17s;
+    }
     double x[3 * NPE];
 #pragma unroll
     for (int k = 0; k < NPE; k++) {
@@ -475,7 +637,7 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
 #pragma unroll
             for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
             local_F<NPE>(x, g, B, Binv, Floc);
-            if (!eval_law(m, Floc, Ta, ts, true, L)) {
+            if (!eval_law(m, Floc, Ta, ts, true, L, l17)) {
                 atomicMin(a.corrupt, (int)e);
                 return;
             }
@@ -544,7 +706,7 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
 // ---------------------------------------------------------------------------
 constexpr int kQRec = 51;   // Ft(9) BBt(9) S(9) Z(9) Y|X(9) czz codot ceq d00 d11 mfs
 
-template <int NPE, int NQP, typename TT, int NSPLIT, int BSPLIT = 1, int GROUPS = 1>
+template <int NPE, int NQP, typename TT, int NSPLIT, int BSPLIT = 1, int GROUPS = 1, bool LAND17 = false>
 __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(const ElemArgs a0) {
     constexpr int EPW = kTanElemsPerWave;
     constexpr int EPG = EPW * GROUPS;     // elements per workgroup
@@ -560,12 +722,20 @@ __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(co
     const int64_t e0 = (int64_t)bid * EPG;
     // ---- stage A ----
     if (lane < EPG * NQP) {
-        const int slot = lane / NQP, q = lane - NQP * (lane / NQP);
+        // element index fastest: the SoA loads (dN/dX, bases, connectivity) of consecutive
+        // lanes hit consecutive elements of one row
+        const int q = lane / EPG, slot = lane - EPG * (lane / EPG);
         const int64_t e = e0 + slot;
         if (e < E) {
             const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
             double ts;
-    const double Ta = tension_of(m, a.time, a.tact[e], ts);
+            const double Ta = tension_of(m, a.time, a.tact[e], ts);
+            L17 l17s;
+            const L17* l17 = nullptr;
+            if (LAND17 && m.tension_fn == CMG_FN_LAND17) {
+                load_l17(a, e, a.matidx ? a.matidx[e] : 0, m, l17s);
+                l17 = &l17s;
+            }
             double x[3 * NPE];
 #pragma unroll
             for (int k = 0; k < NPE; k++) {
@@ -581,7 +751,7 @@ __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(co
             for (int r = 0; r < 9; r++) B[r] = a.bases[(int64_t)(q * 9 + r) * E + e];
             local_F<NPE>(x, g, B, Binv, Floc);
             Law L;
-            if (!eval_law(m, Floc, Ta, ts, true, L)) atomicMin(a.corrupt, (int)e);
+            if (!eval_law(m, Floc, Ta, ts, true, L, l17)) atomicMin(a.corrupt, (int)e);
             {
 #pragma clang fp contract(off)
                 double FS[9], St[9], BinvT[9], Bt[9], P1[9], P[9];
@@ -1067,8 +1237,13 @@ void launch_element_export(int npe, const ElemArgs& a, const double* Xref, doubl
 void launch_element_residual(int npe, const ElemArgs& a, hipStream_t s) {
     if (a.E == 0) return;
     const int nblk = (int)((a.E + kResidualBlock - 1) / kResidualBlock);
-    if (npe == 10)
+    const bool l17 = a.l17_params != nullptr;
+    if (npe == 10 && l17)
+        hipLaunchKernelGGL((k_residual<10, 4, true>), dim3(nblk, a.n_sets), dim3(kResidualBlock), 0, s, a);
+    else if (npe == 10)
         hipLaunchKernelGGL((k_residual<10, 4>), dim3(nblk, a.n_sets), dim3(kResidualBlock), 0, s, a);
+    else if (l17)
+        hipLaunchKernelGGL((k_residual<4, 1, true>), dim3(nblk, a.n_sets), dim3(kResidualBlock), 0, s, a);
     else
         hipLaunchKernelGGL((k_residual<4, 1>), dim3(nblk, a.n_sets), dim3(kResidualBlock), 0, s, a);
 }
@@ -1080,7 +1255,13 @@ void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipS
 #define CMG_LAUNCH(K) hipLaunchKernelGGL(K, dim3(nblk, a.n_sets), dim3(64), 0, s, a)
 #define CMG_LAUNCH2(K) hipLaunchKernelGGL(K, dim3(nblk, a.n_sets), dim3(128), 0, s, a)
 #define CMG_LAUNCH_G2(K, T) hipLaunchKernelGGL(K, dim3(nblk2, a.n_sets), dim3(T), 0, s, a)
-    if (npe == 10) {
+    if (a.l17_params) {
+        // Land17 contexts: one shape per element type (the tuned variants stay free of its code)
+        if (npe == 10 && fp32) CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, float, 1, 2, 2, true>), 256);
+        else if (npe == 10) CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, double, 1, 2, 2, true>), 256);
+        else if (fp32) CMG_LAUNCH((k_residual_tangent_v2<4, 1, float, 1, 1, 1, true>));
+        else CMG_LAUNCH((k_residual_tangent_v2<4, 1, double, 1, 1, 1, true>));
+    } else if (npe == 10) {
         if (variant == 5) {
             if (fp32) CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, float, 1, 1, 2>), 128);
             else CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, double, 1, 1, 2>), 128);
@@ -1121,6 +1302,15 @@ void launch_surface(const SurfArgs& a, bool tangent, hipStream_t s) {
         hipLaunchKernelGGL((k_surface<3>), dim3(nblk, a.n_sets), dim3(64), 0, s, a, tangent);
     else
         hipLaunchKernelGGL((k_surface<6>), dim3(nblk, a.n_sets), dim3(64), 0, s, a, tangent);
+}
+
+void launch_land17_commit(int npe, const ElemArgs& a, double* state, hipStream_t s) {
+    if (a.E == 0) return;
+    const dim3 grid((unsigned)((a.E + 127) / 128));
+    if (npe == 10)
+        hipLaunchKernelGGL((k_land17_commit<10, 4>), grid, dim3(128), 0, s, a, state);
+    else
+        hipLaunchKernelGGL((k_land17_commit<4, 1>), grid, dim3(128), 0, s, a, state);
 }
 
 void launch_t10_shape(int64_t E, const int32_t* conn_aos, const double* X, double* dNdX_soa, double* w,

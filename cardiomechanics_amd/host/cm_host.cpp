@@ -353,8 +353,10 @@ void TetgenModel::LoadActivationTimes(const std::string& lat) {
 // materials (CBConstitutiveModelGuccione.cpp:20-71, CBConstitutiveModelHolzapfel.cpp:18-85,
 // CBMaterial.cpp:19-74, CBTensionFromFunction.cpp:36-56)
 // ===========================================================================
-std::vector<cmg_material> MaterialsFromParameters(const ParameterMap& p, const std::vector<int32_t>& ids) {
+std::vector<cmg_material> MaterialsFromParameters(const ParameterMap& p, const std::vector<int32_t>& ids,
+                                                  std::vector<std::vector<double>>* land17) {
     std::vector<cmg_material> out;
+    if (land17) land17->assign(ids.size(), std::vector<double>());
     for (int32_t id : ids) {
         const std::string own = "Materials.Mat_" + std::to_string(id) + ".";
         const std::string def = "Materials.Mat_Default.";
@@ -409,8 +411,48 @@ std::vector<cmg_material> MaterialsFromParameters(const ParameterMap& p, const s
             if (tkey("kss", 0.0) != 0.0 || tkey("knn", 0.0) != 0.0 || tkey("ksn", 0.0) != 0.0)
                 throw std::runtime_error("TanH stress coefficients kss/knn/ksn are not on this path (only kff)");
             m.tension_fn = CMG_FN_TANH;
+        } else if (tm == "Land17") {
+            // CBTensionModelLand17::InitParamsFromXml (CBTensionModelLand17.cpp:432-500); Tmax is the
+            // material's TensionMax with its Mat_Default fallback (CBMaterial.cpp:39-42)
+            m.tension_max = p.Get<double>(key("TensionMax"), 0.0);
+            auto lkey = [&](const std::string& tail, double dflt) {
+                const std::string o = own + "Land17." + tail, d = def + "Land17." + tail;
+                // InitKey reads the fallback key without its tail (:289-295): no value
+                if (!p.IsAvailable(o) && p.IsAvailable(d)) return p.Get<double>(def + "Land17");
+                return p.Get<double>(o, dflt);
+            };
+            if (lkey("kss", 0.0) != 0.0 || lkey("knn", 0.0) != 0.0 || lkey("ksn", 0.0) != 0.0)
+                throw std::runtime_error("Land17 stress coefficients kss/knn/ksn are not on this path (only kff)");
+            // block order of cmg_set_land17_parameters; the two switches have no fallback (:434, :445)
+            const char* names[22] = {"TRPNn", "TRPNk", "Ca50", "ku", "nTM", "TRPN50", "kuw", "kws",
+                                     "rs", "rw", "gammas", "gammaw", "phi", "Aeff", "beta0", "beta1",
+                                     "Tref", "a", "k", "eta_l", "eta_s", "xi"};
+            const double dflts[22] = {2.0, 0.1, 0.805, 1.0, 5.0, 0.35, 0.182, 0.012, 0.25, 0.5, 0.0085,
+                                      0.615, 2.23, 25.0, 2.3, -2.4, 120.0, 2.1, 7.0, 200.0, 20.0, 1.0};
+            std::vector<double> b(CMG_LAND17_NPARAMS, 0.0);
+            for (int k = 0; k < 22; k++) b[k] = lkey(names[k], dflts[k]);
+            b[22] = lkey("startTime", 0.0);
+            b[23] = lkey("cycleLength", 1.0);
+            const std::string ca = p.Get<std::string>(own + "Land17.CalciumTransientType", "Coppini");
+            if (ca == "Coppini") b[24] = 0.0;
+            else if (ca == "Brixius") b[24] = 1.0;
+            else if (ca == "External" || ca == "Elphy")
+                throw std::runtime_error("Land17: calcium transient " + ca + " needs a calcium source this build "
+                                         "does not have (Coppini, Brixius)");
+            else
+                throw std::runtime_error("CBTensionModelLand17::InitParamsFromXml: no suitable calcium transient "
+                                         "was defined by the user");
+            // any value other than OFF switches the rate dependence on (:368)
+            b[25] = p.Get<std::string>(own + "Land17.rateDependancy", "OFF") == "OFF" ? 0.0 : 1.0;
+            b[26] = lkey("kff", 1.0);
+            const char* st[7] = {"XS", "XW", "TRPN", "TmBlocked", "ZetaS", "ZetaW", "Cd"};
+            const double st0[7] = {0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < 7; k++) b[27 + k] = lkey(st[k], st0[k]);
+            m.tension_fn = CMG_FN_LAND17;
+            if (!land17) throw std::runtime_error("Land17 materials need the parameter-block output");
+            (*land17)[out.size()] = b;
         } else if (tm != "None") {
-            throw std::runtime_error("tension model " + tm + " is not on this path (FromFunction, TanH)");
+            throw std::runtime_error("tension model " + tm + " is not on this path (FromFunction, TanH, Land17)");
         }
         out.push_back(m);
     }
@@ -560,7 +602,8 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
         }
         matidx[e] = (int32_t)(it - ids.begin());
     }
-    std::vector<cmg_material> mats = MaterialsFromParameters(p, ids);
+    std::vector<std::vector<double>> land17;
+    std::vector<cmg_material> mats = MaterialsFromParameters(p, ids, &land17);
     // pressure surfaces: surface indices named by the plugin groups
     std::vector<int32_t> sel_conn, sel_group;
     std::set<int> wanted;
@@ -631,6 +674,9 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
     mesh.surfaces = sel_conn.empty() ? nullptr : sel_conn.data();
     mesh.surface_group = sel_group.empty() ? nullptr : sel_group.data();
     check(cmg_create(&mesh, mats.data(), (int32_t)mats.size(), &o, &ctx_), "cmg_create");
+    for (size_t k = 0; k < land17.size(); k++)
+        if (!land17[k].empty())
+            check(cmg_set_land17_parameters(ctx_, (int32_t)k, land17[k].data()), "cmg_set_land17_parameters");
     int64_t nn, ne;
     check(cmg_sizes(ctx_, &nn, &ne, &nDofs_, &nnzb_), "cmg_sizes");
     formulation_ = std::make_unique<CBFormulationTotalLagrangianGPU>(this);

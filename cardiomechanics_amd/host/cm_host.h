@@ -357,6 +357,9 @@ private:
     std::string exportFile_;
 };
 
-std::vector<cmg_material> MaterialsFromParameters(const ParameterMap& p, const std::vector<int32_t>& ids);
+// land17 (optional) receives, per material, the Land17 parameter block of
+// cmg_set_land17_parameters, empty for materials of another tension model
+std::vector<cmg_material> MaterialsFromParameters(const ParameterMap& p, const std::vector<int32_t>& ids,
+                                                  std::vector<std::vector<double>>* land17 = nullptr);
 
 }  // namespace cmhost

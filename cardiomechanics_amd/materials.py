@@ -20,6 +20,7 @@ LAW_GUCCIONE = 0
 LAW_HOLZAPFEL = 1
 FN_NONE, FN_LINEAR, FN_SINUS, FN_SINUSDRIVER, FN_DOUBLEHILL = 0, 1, 2, 3, 4
 FN_TANH = 5   # the TanH tension model (a stretch-dependent law, not a time function)
+FN_LAND17 = 6  # the Land17 cross-bridge model (ODE state per element)
 _FN_NAMES = {"None": FN_NONE, "Linear": FN_LINEAR, "Sinus": FN_SINUS, "SinusDriver": FN_SINUSDRIVER,
              "DoubleHill": FN_DOUBLEHILL}
 
@@ -36,6 +37,8 @@ class Material:
     # startTime, stopTime, keepMaxValue, omega, phi, DoubleHill period, contrTimeOffset,
     # relaxTimeOffset, contrRateConst, relaxRateConst, onsetTime, driverMax
     fn: list = field(default_factory=lambda: [0.0] * 12)
+    # Land17 parameter block (LAND17_KEYS order), set by with_land17
+    land17: list | None = None
 
 
 def guccione(C, bf, bt, bfs, K, density=1082.0, a_scale=1.0, b_scale=1.0, **kw) -> Material:
@@ -70,6 +73,61 @@ def with_tanh(m: Material, **kw) -> Material:
     """the material with the TanH tension model (Tmax = 1000, TanH parameters as keywords)"""
     m.tension_max = 1000.0
     m.tension_fn, m.fn = tanh_params(**kw)
+    return m
+
+
+# Land17 keys and defaults in the order of the device parameter block (InitParamsFromXml,
+# CBTensionModelLand17.cpp:428-500): model constants, then startTime, cycleLength, the
+# calcium transient (0 Coppini, 1 Brixius), rateDependancy (0 OFF, 1 ON), kff, then the
+# initial state XS XW TRPN TmBlocked ZetaS ZetaW Cd
+LAND17_KEYS = (("TRPNn", 2.0), ("TRPNk", 0.1), ("Ca50", 0.805), ("ku", 1.0), ("nTM", 5.0), ("TRPN50", 0.35),
+               ("kuw", 0.182), ("kws", 0.012), ("rs", 0.25), ("rw", 0.5), ("gammas", 0.0085), ("gammaw", 0.615),
+               ("phi", 2.23), ("Aeff", 25.0), ("beta0", 2.3), ("beta1", -2.4), ("Tref", 120.0), ("a", 2.1),
+               ("k", 7.0), ("eta_l", 200.0), ("eta_s", 20.0), ("xi", 1.0), ("startTime", 0.0),
+               ("cycleLength", 1.0), ("CalciumTransientType", 0.0), ("rateDependancy", 0.0), ("kff", 1.0),
+               ("XS", 0.0), ("XW", 0.0), ("TRPN", 0.0), ("TmBlocked", 1.0), ("ZetaS", 0.0), ("ZetaW", 0.0),
+               ("Cd", 0.0))
+LAND17_NPARAMS = 40
+_CALCIUM = {"Coppini": 0.0, "Brixius": 1.0}
+
+
+def land17_params(**kw):
+    """Land17 parameter block [40] (LAND17_KEYS, keyword = key name; CalciumTransientType
+    'Coppini' or 'Brixius', rateDependancy 'ON'/'OFF' or 1/0)"""
+    vals = dict(LAND17_KEYS)
+    for k, v in kw.items():
+        if k not in vals:
+            raise KeyError(f"unknown Land17 parameter {k}")
+        if k == "CalciumTransientType":
+            if v not in _CALCIUM:
+                raise ValueError("Land17: CalciumTransientType must be Coppini or Brixius (External and Elphy "
+                                 "need a calcium source this build does not have)")
+            v = _CALCIUM[v]
+        elif k == "rateDependancy" and isinstance(v, str):
+            v = 1.0 if v.upper() == "ON" else 0.0
+        vals[k] = float(v)
+    out = [vals[k] for k, _ in LAND17_KEYS]
+    return out + [0.0] * (LAND17_NPARAMS - len(out))
+
+
+def land17_calcium_tables():
+    """the Coppini and Brixius calcium transients the device uses (csrc/land17_calcium.h)"""
+    import os
+    import re
+
+    text = open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "land17_calcium.h")).read()
+    body = text[text.index("kLand17Cai[2][kLand17CaSamples] = {"):]
+    blocks = body.split("},")
+    vals = [[float(v) for v in re.findall(r"\d+\.\d+", b)] for b in blocks[:2]]
+    return vals[0], vals[1]
+
+
+def with_land17(m: Material, tension_max=1000.0, **kw) -> Material:
+    """the material with the Land17 tension model (Tmax converts its kPa to Pa: 1000)"""
+    m.tension_max = float(tension_max)
+    m.tension_fn = FN_LAND17
+    m.fn = [0.0] * 12
+    m.land17 = land17_params(**kw)
     return m
 
 
@@ -184,8 +242,28 @@ def from_parameters(params: dict, mat_index) -> Material:
             contr_offset=g("DoubleHill.ContrTimeOffset", 0.215), relax_offset=g("DoubleHill.RelaxTimeOffset", 0.362),
             contr_rate=g("DoubleHill.ContrRateConst", 1.32), relax_rate=g("DoubleHill.RelaxRateConst", 21.9),
             onset=g("DoubleHill.OnsetTime", 0.0))
+    elif tmodel == "Land17":
+        # CBTensionModelLand17::InitParamsFromXml (CBTensionModelLand17.cpp:432-500); Tmax is
+        # the material's TensionMax with its Mat_Default fallback (CBMaterial.cpp:39-42)
+        own, dflt = f"Materials.Mat_{mat_index}.Land17", "Materials.Mat_Default.Land17"
+
+        def key(tail, d):
+            if own + "." + tail not in params and dflt + "." + tail in params:
+                # InitKey reads the fallback key without its tail (:289-295): no value, Get throws
+                raise KeyError(dflt)
+            return float(params.get(own + "." + tail, d))
+
+        for k in ("kss", "knn", "ksn"):
+            if key(k, 0.0) != 0.0:
+                raise ValueError("Land17 stress coefficients kss/knn/ksn are not on this path (only kff)")
+        # the two switches have no Mat_Default fallback (:434, :445)
+        vals = {k: key(k, d) for k, d in LAND17_KEYS if k not in ("CalciumTransientType", "rateDependancy")}
+        vals["CalciumTransientType"] = str(params.get(own + ".CalciumTransientType", "Coppini"))
+        rate = str(params.get(own + ".rateDependancy", "OFF"))
+        vals["rateDependancy"] = 0.0 if rate == "OFF" else 1.0   # any other value switches it on (:368)
+        with_land17(m, tension_max=float(_get(params, mat_index, "TensionMax", 0.0)), **vals)
     elif tmodel != "None":
-        raise ValueError(f"tension model {tmodel} is not on this path (FromFunction, TanH)")
+        raise ValueError(f"tension model {tmodel} is not on this path (FromFunction, TanH, Land17)")
     return m
 
 

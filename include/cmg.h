@@ -49,6 +49,10 @@ extern "C" {
 #define CMG_FN_SINUSDRIVER 3
 #define CMG_FN_DOUBLEHILL 4
 #define CMG_FN_TANH 5   /* the TanH tension model (CBTensionModelTanH.cpp), not a time function */
+#define CMG_FN_LAND17 6 /* the Land17 cross-bridge model (CBTensionModelLand17.cpp): per-element ODE
+                           state, parameters through cmg_set_land17_parameters */
+
+#define CMG_LAND17_NPARAMS 40
 
 #define CMG_PREC_FP64 0
 #define CMG_PREC_MIXED 1
@@ -120,7 +124,8 @@ typedef struct cmg_options {
                                    two node passes, 4 the same with two lanes per element and
                                    spatial pair, 1 per-lane quadrature points, 5 / 6 variants 2 / 4
                                    with 14 elements per workgroup; 0 (default): 6 in FP64, 5 in
-                                   mixed precision */
+                                   mixed precision.  A context with a Land17 material always
+                                   runs the shape of variant 6 */
     int32_t krylov_method;      /* 0 GMRES(restart), 1 BiCGStab with device-side scalars and graph-replayed
                                    iterations, 2 BiCGStab with host-side scalars (default); all right
                                    block-Jacobi preconditioned */
@@ -311,6 +316,23 @@ int cmg_newmark_energies(cmg_ctx* ctx, double* kinetic, double* damping_dissipat
 int cmg_newmark_solve(cmg_ctx* ctx, double time, double dt, int32_t guess, cmg_step_stats* stats);
 int cmg_trial_volumes(cmg_ctx* ctx, double* volumes /* [n_groups] */);
 int cmg_commit_step(cmg_ctx* ctx);
+
+/* ---- Land17 tension model (CBTensionModelLand17.cpp) -------------------------
+ * A material with tension_fn = CMG_FN_LAND17 carries one set of cross-bridge states per
+ * element (XS XW TRPN TmBlocked ZetaS ZetaW Cd lambda): every evaluation at time t
+ * advances them by one explicit Euler step of (t - t_commit) from the committed state,
+ * at each quadrature point's fibre stretch (CalcActiveTension, :298-372), and the
+ * accepted static or Newmark step commits the state of the element's last quadrature
+ * point (SaveStateVariablesAsNeeded, :274-293).  Active PK2 kff T / lambda_f on ff.
+ * params[40]: TRPNn TRPNk Ca50 ku nTM TRPN50 kuw kws rs rw gammas gammaw phi Aeff beta0
+ * beta1 Tref a k eta_l eta_s xi (0-21, InitParamsFromXml :428-500), startTime
+ * cycleLength, calcium transient (0 Coppini, 1 Brixius), rateDependancy (0 OFF, 1 ON),
+ * kff (22-26), initial XS XW TRPN TmBlocked ZetaS ZetaW Cd (27-33); the reference's
+ * defaults until set.  Setting a material's parameters resets its elements' state. */
+int cmg_set_land17_parameters(cmg_ctx* ctx, int32_t material, const double* params);
+/* state [8][E] (row k of element e at k*E + e) and the time of the last commit */
+int cmg_get_land17_state(cmg_ctx* ctx, double* state, double* commit_time);
+int cmg_set_land17_state(cmg_ctx* ctx, const double* state, double commit_time);
 
 /* ---- Parameter-set batch ----------------------------------------------------
  * No counterpart in the reference, which solves one parameter set per run: S

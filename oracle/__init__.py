@@ -36,6 +36,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 LAW_GUCCIONE = 0
 LAW_HOLZAPFEL = 1
 FN_NONE, FN_LINEAR, FN_SINUS, FN_SINUSDRIVER, FN_DOUBLEHILL = 0, 1, 2, 3, 4
+FN_LAND17 = 6
 SUCCESS, FAILED, CORRUPT_ELEMENT = 2, -3, -6
 
 
@@ -86,6 +87,12 @@ def lib():
             "orc_cpu_newmark_step": (ctypes.c_int32, [ctypes.c_void_p] + [ctypes.c_double] * 8 + [ctypes.c_int32, _D]),
             "orc_cpu_static_step": (ctypes.c_int32, [ctypes.c_void_p] + [ctypes.c_double] * 6 + [ctypes.c_int32, _D]),
             "orc_fd_epsilon": (ctypes.c_double, [_D, ctypes.c_int64, ctypes.c_double]),
+            "orc_land17_configure": (None, [_D, _D, ctypes.c_double]),
+            "orc_land17_cai": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
+            "orc_land17_step": (ctypes.c_double, [_D, ctypes.c_double, _D, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_double, _D, _D]),
+            "orc_land17_commit_batch": (None, [ctypes.c_int32, ctypes.c_int64, _I32, _D, _D, _D, _M, _I32, _D, _D,
+                                               ctypes.c_double, ctypes.c_double, _D]),
             "orc_mass_batch": (None, [ctypes.c_int32, ctypes.c_int64, _I32, _D, _D, _U8, ctypes.c_int32, _D]),
             "orc_t10_residual": (ctypes.c_int32, [_D, _D, _D, ctypes.c_double, _U8, _M, ctypes.c_double,
                                                   ctypes.c_double, _D]),
@@ -430,3 +437,56 @@ def tangent_batch(ed: ElementData, coords, mats, time=0.0, kind="cs", eps=1e-12,
     else:
         rc = lib().orc_cs_tangent_batch(*args, Ke.ctypes.data_as(_D), st.ctypes.data_as(_I32))
     return rc, Ke, st
+
+
+# ---- Land17 (CBTensionModelLand17.cpp) -----------------------------------------
+def land17_step(params, tmax, state, lam, dt_ms, cai):
+    """one evaluation with an explicit Cai: (tension, next state [8], dT/dlambda)"""
+    p, pp = _d(params)
+    st, pst = _d(state)
+    nxt = np.zeros(8)
+    d = np.zeros(1)
+    t = lib().orc_land17_step(pp, tmax, pst, lam, dt_ms, cai, nxt.ctypes.data_as(_D), d.ctypes.data_as(_D))
+    return t, nxt, float(d[0])
+
+
+def land17_cai(params, time, tact):
+    p, pp = _d(params)
+    return lib().orc_land17_cai(pp, time, tact)
+
+
+class Land17State:
+    """Per-element Land17 state [E][8] of a mesh (XS XW TRPN TmBlocked ZetaS ZetaW Cd lambda)
+    and the time of its last commit; configure() hands it to the element batches for an
+    evaluation at `time`, commit() advances it after an accepted step."""
+
+    def __init__(self, mesh, materials):
+        self.mats = list(materials)
+        E = mesh.n_elements
+        self.params = np.zeros((len(self.mats), 40))
+        for i, m in enumerate(self.mats):
+            if getattr(m, "land17", None) is not None:
+                self.params[i] = m.land17
+        self.state = np.zeros((E, 8))
+        mi = np.asarray(mesh.material, dtype=np.int64)
+        self.state[:, :7] = self.params[mi, 27:34]
+        self.state[:, 7] = 1.0           # lambda of the undeformed configuration
+        self.t_prev = 0.0
+        self.active = any(getattr(m, "land17", None) is not None for m in self.mats)
+
+    def configure(self, time):
+        if self.active:
+            lib().orc_land17_configure(self.params.ctypes.data_as(_D), self.state.ctypes.data_as(_D),
+                                       (time - self.t_prev) * 1000.0)
+
+    def commit(self, ed, coords, time):
+        if not self.active:
+            return
+        coords, pc = _d(coords)
+        marr = materials_array(self.mats)
+        lib().orc_land17_commit_batch(ed.npe, ed.conn.shape[0], ed.conn.ctypes.data_as(_I32), pc,
+                                      ed.dNdX.ctypes.data_as(_D), ed.bases.ctypes.data_as(_D), marr,
+                                      ed.matidx.ctypes.data_as(_I32), ed.tact.ctypes.data_as(_D),
+                                      self.params.ctypes.data_as(_D), time, (time - self.t_prev) * 1000.0,
+                                      self.state.ctypes.data_as(_D))
+        self.t_prev = time

@@ -56,6 +56,8 @@ class CpuMechanics:
         self.time_last = 0.0
         self.step_last = 0.0
         self.guess = np.zeros(self.n)
+        # Land17 cross-bridge state per element, committed after every accepted step
+        self.land17 = O.Land17State(mesh, self.materials)
 
     # -- element / surface loops ------------------------------------------------
     def _mass(self, consistent, chunk=50_000):
@@ -99,6 +101,7 @@ class CpuMechanics:
         return f
 
     def residual(self, x, time, include_pressure=True):
+        self.land17.configure(time)
         rc, fe, st = O.residual_batch(self.ed, x.reshape(-1, 3), self.materials, time)
         if rc != O.SUCCESS:
             raise FloatingPointError(f"corrupt element {int(np.argmax(st != O.SUCCESS))}")
@@ -109,6 +112,7 @@ class CpuMechanics:
         return f
 
     def element_tangents(self, x, time):
+        self.land17.configure(time)
         rc, Ke, st = O.tangent_batch(self.ed, x.reshape(-1, 3), self.materials, time, kind="cs")
         if rc != O.SUCCESS:
             raise FloatingPointError("corrupt element")
@@ -223,9 +227,10 @@ class CpuMechanics:
         self.time_last, self.step_last = time, dt
         self.d, it, rn = self.newton(time, self.d, tol=tol)
         self.nodes = self.nodes + self.d
+        self.land17.commit(self.ed, self.nodes.reshape(-1, 3), time)
         return it, rn
 
-    def newmark_solve(self, time, dt, beta=0.25, gamma=0.5, tol=1e-12, from_trial=False):
+    def newmark_solve(self, time, dt, beta=0.25, gamma=0.5, tol=1e-12, from_trial=False, atol=1e-12):
         """predictor + Newton (CBSolverNewmarkBeta.cpp:845-923); the increment stays
         pending until commit_step"""
         dpred = dt * self.vel + dt * dt * 0.5 * (1 - 2 * beta) * self.acc
@@ -237,8 +242,8 @@ class CpuMechanics:
         damp = None
         if self.rayleigh != (0.0, 0.0):
             damp = (self.damping_matrix(time), gamma / (beta * dt), vpred)
-        d, it, rn = self.newton(time, d0, cm=cm, dpred=dpred, tol=tol, damp=damp)
-        self._pending = dict(d=d, dpred=dpred, vpred=vpred, dt=dt, beta=beta, gamma=gamma,
+        d, it, rn = self.newton(time, d0, cm=cm, dpred=dpred, tol=tol, atol=atol, damp=damp)
+        self._pending = dict(d=d, dpred=dpred, vpred=vpred, dt=dt, beta=beta, gamma=gamma, time=time,
                              C=damp[0] if damp is not None else None)
         return it, rn
 
@@ -260,10 +265,11 @@ class CpuMechanics:
             self.damping_energy += float((p["C"] @ self.vel) @ p["d"])
         self.d = p["d"]
         self.nodes = self.nodes + p["d"]
+        self.land17.commit(self.ed, self.nodes.reshape(-1, 3), p["time"])
         self._pending = None
 
-    def newmark_step(self, time, dt, beta=0.25, gamma=0.5, tol=1e-12):
-        it, rn = self.newmark_solve(time, dt, beta, gamma, tol)
+    def newmark_step(self, time, dt, beta=0.25, gamma=0.5, tol=1e-12, atol=1e-12):
+        it, rn = self.newmark_solve(time, dt, beta, gamma, tol, atol=atol)
         self.commit_step()
         return it, rn
 

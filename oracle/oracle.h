@@ -37,6 +37,8 @@ enum {
     ORC_FN_SINUS = 2,
     ORC_FN_SINUSDRIVER = 3,
     ORC_FN_DOUBLEHILL = 4,
+    ORC_FN_LAND17 = 6,       /* Land17 cross-bridge model; parameters and per-element state through
+                                orc_land17_configure */
     ORC_FN_TANH = 5          /* TanH tension model; fn = cycleLength, l0, t_emd, S_peak, t_dur,
                                 tau_c0, ld, ld_up, tau_r, kff, startTime */
 };
@@ -159,6 +161,16 @@ int32_t orc_cpu_newmark_step(orc_cpu_solver* s, double time, double dt, double b
 int32_t orc_cpu_static_step(orc_cpu_solver* s, double time, double dt, double eps, double atol, double rtol,
                             double stol, int32_t maxit, double* stats);
 double orc_fd_epsilon(const double* a, int64_t n, double eps);
+
+/* ---- Land17 tension model (CBTensionModelLand17.cpp) -------------------- */
+void orc_land17_configure(const double* params, const double* state, double dt_ms);
+double orc_land17_cai(const double* p, double time, double tact);
+double orc_land17_step(const double* p, double tmax, const double* state, double lambda, double dt_ms, double cai,
+                       double* next, double* dTdl);
+void orc_land17_commit_batch(int32_t npe, int64_t n_elem, const int32_t* conn, const double* coords,
+                             const double* dNdX, const double* bases, const orc_material* mats,
+                             const int32_t* matidx, const double* tact, const double* params, double time,
+                             double dt_ms, double* state);
 
 #ifdef __cplusplus
 }

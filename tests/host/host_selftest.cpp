@@ -52,6 +52,39 @@ int main(int argc, char** argv) {
     CHECK(std::fabs(mats[0].fn[3] - 20.943951) < 1e-12);
     CHECK(mats[1].law == CMG_LAW_HOLZAPFEL && mats[1].params[8] == 1e6 && mats[1].params[9] == 100.0);
 
+    // Land17: per-key Mat_Default fallback reads the tail-less key (no value) and throws;
+    // the calcium switch has no fallback; TensionMax falls back like every material key
+    {
+        ParameterMap q;
+        q.ParseXMLString(R"(<settings><Materials>
+  <Mat_Default><TensionMax>1000</TensionMax><Land17><rateDependancy>ON</rateDependancy></Land17></Mat_Default>
+  <Mat_2><Type>Guccione</Type><Guccione><C>880</C><bf>8</bf><bt>6</bt><bfs>12</bfs><K>1e6</K></Guccione>
+    <TensionModel>Land17</TensionModel>
+    <Land17><Tref>100</Tref><CalciumTransientType>Brixius</CalciumTransientType><TmBlocked>0.5</TmBlocked></Land17></Mat_2>
+  <Mat_3><Type>Guccione</Type><Guccione><C>880</C><bf>8</bf><bt>6</bt><bfs>12</bfs><K>1e6</K></Guccione></Mat_3>
+</Materials></settings>)");
+        std::vector<std::vector<double>> l17;
+        auto lm = MaterialsFromParameters(q, {2, 3}, &l17);
+        CHECK(lm[0].tension_fn == CMG_FN_LAND17 && lm[0].tension_max == 1000.0);
+        CHECK(l17.size() == 2 && l17[0].size() == CMG_LAND17_NPARAMS && l17[1].empty());
+        CHECK(l17[0][16] == 100.0 && l17[0][0] == 2.0 && l17[0][24] == 1.0 && l17[0][26] == 1.0);
+        CHECK(l17[0][25] == 0.0);   // rateDependancy is read from the material's own key only
+        CHECK(l17[0][30] == 0.5 && l17[0][29] == 0.0);
+        ParameterMap r;
+        r.ParseXMLString(R"(<settings><Materials>
+  <Mat_Default><Land17><Tref>90</Tref></Land17></Mat_Default>
+  <Mat_2><Type>Guccione</Type><Guccione><C>880</C><bf>8</bf><bt>6</bt><bfs>12</bfs><K>1e6</K></Guccione>
+    <TensionModel>Land17</TensionModel><TensionMax>1000</TensionMax></Mat_2>
+</Materials></settings>)");
+        bool threw = false;
+        try {
+            MaterialsFromParameters(r, {2}, &l17);
+        } catch (const std::exception&) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+
     // time functions
     auto lin = CBAnalyticFunction::Create("Linear", p, "Plugins.ApplyPressureFromFunction.Groups.G1.Intervals.Interval_1");
     CHECK(std::fabs((*lin)(0.25) - 0.25) < 1e-15);

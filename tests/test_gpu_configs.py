@@ -258,7 +258,7 @@ def test_lv1m_element_kernel_and_warm_started_newmark_steps():
             net.record_before_step(t, rate)
         return None
 
-    recs = newmark_window(mesh, 0, tol, 0.0045, warm=warm, on_record=record)
+    recs = newmark_window(mesh, 0, tol, 0.004, warm=warm, on_record=record)
     if state["net"] is not None:
         state["net"].close()
     progress("window done")

@@ -453,3 +453,84 @@ def test_lv_beat_passes_activation_onset(precision):
             assert rn < 1e-8 * r0, (t, dt, rn, reason, r0)
         else:
             assert rn < 1e-6, (t, dt, rn, reason)
+
+
+def land17_case(rate="ON"):
+    mesh = M.lv_mesh(dims=(2, 4, 8))
+    m = MT.with_land17(MT.lv_holzapfel(), rateDependancy=rate, CalciumTransientType="Brixius")
+    rng = np.random.default_rng(17)
+    E = mesh.n_elements
+    st = np.empty((E, 8))
+    st[:, 0] = rng.uniform(0.0, 0.1, E)       # XS
+    st[:, 1] = rng.uniform(0.0, 0.1, E)       # XW
+    st[:, 2] = rng.uniform(0.1, 0.6, E)       # TRPN
+    st[:, 3] = rng.uniform(0.3, 0.9, E)       # TmBlocked
+    st[:, 4] = rng.uniform(-0.05, 0.05, E)    # ZetaS
+    st[:, 5] = rng.uniform(-0.05, 0.05, E)    # ZetaW
+    st[:, 6] = rng.uniform(0.0, 0.02, E)      # Cd
+    st[:, 7] = rng.uniform(0.98, 1.05, E)     # lambda at the commit
+    return mesh, [m], st
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_land17_element_kernels_match_oracle(precision):
+    """Land17 active tension (CBTensionModelLand17.cpp:298-372) from a committed cross-bridge
+    state per element, rate dependence on: element residuals (<= 1e-12) and tangents (the
+    analytic dT/dlambda against the complex-step oracle, <= 1e-12 FP64, 2e-6 mixed).  A
+    context with a Land17 material runs one kernel shape whatever element_variant says."""
+    mesh, mats, st = land17_case()
+    x = deformed(mesh)
+    t_commit, t = 0.05, 0.0515
+    with MechanicsContext(mesh, mats, precision=precision) as ctx:
+        ctx.set_land17_state(st.T.copy(), t_commit)
+        fg = ctx.element_residuals(x, t)
+        Kg = ctx.element_tangents(x, t).astype(np.float64)
+        got, tc = ctx.land17_state()
+    assert tc == t_commit and np.array_equal(got, st.T)
+    l17 = O.Land17State(mesh, mats)
+    l17.state[:] = st
+    l17.t_prev = t_commit
+    l17.configure(t)
+    rc, fo, _ = O.residual_batch(oracle_elements(mesh), x, mats, t)
+    rc2, Ko, _ = O.tangent_batch(oracle_elements(mesh), x, mats, t, kind="cs")
+    assert rc == O.SUCCESS and rc2 == O.SUCCESS
+    ef = (np.abs(fg - fo) / np.abs(fo).max(axis=1, keepdims=True)).max()
+    rel = np.linalg.norm((Kg - Ko).reshape(len(Ko), -1), axis=1) / np.linalg.norm(Ko.reshape(len(Ko), -1), axis=1)
+    print(f"Land17 precision {precision}: residual {ef:.2e}, tangent {rel.max():.2e}")
+    assert ef <= 1e-12
+    assert rel.max() <= (1e-12 if precision == 0 else 2e-6)
+
+
+def test_land17_newmark_steps_and_state_commits_match_oracle():
+    """Newmark (1/4, 1/2, dt = 1 ms) with Land17 tension (Coppini calcium) and cavity
+    pressure: nodes (<= 1e-8) and the committed cross-bridge state (<= 1e-9) after every
+    step against the oracle driver, which commits the state of each element's last
+    quadrature point like the reference's SaveStateVariablesAsNeeded.  The cavity
+    pressure ramps by 100 Pa per step, so every step moves the wall by far more than the
+    Newton tolerances resolve (a sin^2 pulse moves it by micrometres in the first
+    milliseconds, where the two roots at 1e-12 N differ by ~2e-8 of it)."""
+    mesh = M.lv_mesh(dims=(2, 4, 8))
+    mats = [MT.with_land17(MT.lv_holzapfel())]
+    cpu = CpuMechanics(mesh, mats)
+    dt = 1e-3
+    with MechanicsContext(mesh, mats, newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15) as ctx:
+        for k in range(1, 31):
+            t = k * dt
+            ctx.set_pressures([-100.0 * k])
+            cpu.pressures[:] = -100.0 * k
+            rc, st = ctx.newmark_step(t, dt)
+            assert rc == 2, st.as_dict()
+            cpu.newmark_step(t, dt)
+            g = ctx.get_state()["nodes"]
+            u = cpu.nodes - mesh.coords.ravel()
+            err = np.linalg.norm(g - cpu.nodes) / np.linalg.norm(u)
+            sg, tc = ctx.land17_state()
+            so = cpu.land17.state.T
+            serr = np.abs(sg - so).max() / max(np.abs(so[:7]).max(), 1e-300)
+            if k % 5 == 0 or err > 1e-8 or serr > 1e-9:
+                print(f"t={t:.3f} newton={st.newton_iterations} disp err={err:.2e} state err={serr:.2e} "
+                      f"XS max={so[0].max():.2e}")
+            assert tc == pytest.approx(t, abs=1e-15)
+            assert err <= 1e-8
+            assert serr <= 1e-9
+        assert so[0].max() > 0       # cross-bridges formed
