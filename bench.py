@@ -53,6 +53,12 @@ def main():
                     help="Newton abstol, rtol, stol of the beat window runs")
     ap.add_argument("--beat-runs", default="fp64_bj,fp64,mixed,mixed_warm",
                     help="comma list of fp64_bj (block Jacobi), fp64, mixed, mixed_warm (vertex-patch Schwarz)")
+    ap.add_argument("--krylov-method", type=int, default=None,
+                    help="Krylov solver of the beat window runs (0 GMRES, 1 BiCGStab with device-side scalars, "
+                         "2 BiCGStab with host-side scalars; default: the library's)")
+    ap.add_argument("--precond-reuse", type=int, default=None,
+                    help="1: the beat window runs build the preconditioner once per Newton solve (default: the "
+                         "library's)")
     ap.add_argument("--beat-train", type=int, default=2,
                     help="training windows for the warm start (Tmax +-15 %%, activation +-10 ms, seeded)")
     ap.add_argument("--beat-cpu-steps", type=int, default=1,
@@ -192,7 +198,7 @@ def main():
 
     # kernel variants of the fused residual + tangent element kernel; the fastest is the headline
     variants = {}
-    for var in (2, 4, 5, 6, 3, 1):
+    for var in (2, 4, 5, 6, 7, 8, 3, 1):
         ctx = MechanicsContext(mesh, mats, precision=prec, element_variant=var)
         if xd is None:
             xd = DeviceArray(x.ravel())
@@ -318,8 +324,11 @@ def beat_window_runs(args, mesh, mats):
         timing = SV.Timing(0.0, T, dt0)
         res = dict(newton=0, krylov=0, accepted=0, halved=0, failed_attempts=0, kept=0, proposed=0, states={},
                    samples=[])
+        kw = {} if args.krylov_method is None else {"krylov_method": args.krylov_method}
+        if args.precond_reuse is not None:
+            kw["precond_reuse"] = args.precond_reuse
         with MechanicsContext(msh, [mat], precision=prec, preconditioner=pc, newton_atol=atol, newton_rtol=rtol,
-                              newton_stol=stol, newmark_beta=0.25, newmark_gamma=0.5) as ctx:
+                              newton_stol=stol, newmark_beta=0.25, newmark_gamma=0.5, **kw) as ctx:
             net = WarmStartNet(ctx, seed=5) if (weights is not None or record) else None
             if weights is not None:
                 net.set_weights(weights)

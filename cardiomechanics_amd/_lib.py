@@ -61,7 +61,7 @@ class CmgOptions(ctypes.Structure):
                 ("newmark_beta", ctypes.c_double), ("newmark_gamma", ctypes.c_double),
                 ("fd_epsilon", ctypes.c_double), ("rayleigh_alpha", ctypes.c_double),
                 ("rayleigh_beta", ctypes.c_double), ("preconditioner", ctypes.c_int32),
-                ("reserved_i", ctypes.c_int32), ("krylov_eta_max", ctypes.c_double)]
+                ("precond_reuse", ctypes.c_int32), ("krylov_eta_max", ctypes.c_double)]
 
 
 class CmgSolveStats(ctypes.Structure):
@@ -85,7 +85,7 @@ class CmgStepStats(ctypes.Structure):
 class CmgBatchInfo(ctypes.Structure):
     _fields_ = [("n_sets", ctypes.c_int32), ("element_variant", ctypes.c_int32), ("spmv_lanes", ctypes.c_int32),
                 ("cg_chunk", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("tuned", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2), ("element_ms", ctypes.c_double * 6),
+                ("reserved", ctypes.c_int32 * 2), ("element_ms", ctypes.c_double * 8),
                 ("spmv_ms", ctypes.c_double * 3), ("tune_seconds", ctypes.c_double)]
 
     def as_dict(self):

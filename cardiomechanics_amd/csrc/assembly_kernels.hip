@@ -44,19 +44,32 @@ __global__ __launch_bounds__(256) void k_gather_vector(int64_t N, const int64_t*
     f[3 * r + 2] = (m & 4) ? 0.0 : s2;
 }
 
+constexpr int kMassLanes = 16;
+
 __global__ __launch_bounds__(256) void k_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colind,
                                                        const double* mval, double cm, const double* d,
                                                        const double* dp, const uint8_t* bcmask, double* f) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= N) return;
+    // kMassLanes lanes per block row (coalesced reads of the row's mass scalars and
+    // columns), summed over the lanes in a fixed butterfly order
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = t / kMassLanes;
+    const int l = (int)(t - r * kMassLanes);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
-        const int64_t c = colind[k];
-        const double mv = mval[k];
-        s0 += mv * (d[3 * c] - dp[3 * c]);
-        s1 += mv * (d[3 * c + 1] - dp[3 * c + 1]);
-        s2 += mv * (d[3 * c + 2] - dp[3 * c + 2]);
+    if (r < N)
+        for (int64_t k = rowptr[r] + l; k < rowptr[r + 1]; k += kMassLanes) {
+            const int64_t c = colind[k];
+            const double mv = mval[k];
+            s0 += mv * (d[3 * c] - dp[3 * c]);
+            s1 += mv * (d[3 * c + 1] - dp[3 * c + 1]);
+            s2 += mv * (d[3 * c + 2] - dp[3 * c + 2]);
+        }
+#pragma unroll
+    for (int off = kMassLanes / 2; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, kMassLanes);
+        s1 += __shfl_xor(s1, off, kMassLanes);
+        s2 += __shfl_xor(s2, off, kMassLanes);
     }
+    if (r >= N || l != 0) return;
     const uint8_t m = bcmask ? bcmask[r] : 0;
     if (!(m & 1)) f[3 * r] += cm * s0;
     if (!(m & 2)) f[3 * r + 1] += cm * s1;
@@ -123,6 +136,8 @@ __global__ __launch_bounds__(256) void k_gather_matrix(int64_t nnzb, const int64
                                                        VT* vals, bool flip, SetStrides st) {
     const int64_t set = blockIdx.y;
     if (st.active && !st.active[set]) return;
+    // (workgroups in plain order: giving each XCD a contiguous range of slots measured
+    // 19 % slower on the 200k-element LV, with the same HBM fetch)
     int64_t slot;
     if ((int64_t)blockIdx.x < light_blocks) {
         slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -217,7 +232,8 @@ void launch_gather_vector(int64_t N, const int64_t* inc_ptr, const int32_t* inc,
 void launch_add_mass_term(int64_t N, const int64_t* rowptr, const int32_t* colind, const double* mval, double cm,
                           const double* d, const double* dpred, const uint8_t* bcmask, double* f, hipStream_t s) {
     if (N == 0) return;
-    hipLaunchKernelGGL(k_add_mass_term, dim3((N + 255) / 256), dim3(256), 0, s, N, rowptr, colind, mval, cm, d, dpred,
+    hipLaunchKernelGGL(k_add_mass_term, dim3((unsigned)((N * kMassLanes + 255) / 256)), dim3(256), 0, s, N, rowptr,
+                       colind, mval, cm, d, dpred,
                        bcmask, f);
 }
 

@@ -98,7 +98,7 @@ int beval(cmg_batch* b, double time, bool tangent, std::vector<int32_t>& corrupt
             CMG_HIP_CHECK(hipEventCreate(&e0));
             CMG_HIP_CHECK(hipEventCreate(&e1));
             double best = 1e300;
-            for (int v = 1; v <= 6; v++) {
+            for (int v = 1; v <= 8; v++) {
                 launch_element_residual_tangent(c->npe, c->fp32, a, s, v);
                 CMG_HIP_CHECK(hipEventRecord(e0, s));
                 launch_element_residual_tangent(c->npe, c->fp32, a, s, v);
@@ -526,7 +526,7 @@ int cmg_batch_configure(cmg_batch* b, int32_t use_graph, int32_t spmv_lanes, int
     if (!b) return fail("null batch");
     if (spmv_lanes != 0 && spmv_lanes != 4 && spmv_lanes != 8 && spmv_lanes != 16)
         return fail("cmg_batch_configure: spmv_lanes must be 0, 4, 8 or 16");
-    if (element_variant < 0 || element_variant > 6) return fail("cmg_batch_configure: element_variant in 0..6");
+    if (element_variant < 0 || element_variant > 8) return fail("cmg_batch_configure: element_variant in 0..8");
     if (cg_chunk < 1 || cg_chunk > 1024) return fail("cmg_batch_configure: cg_chunk in 1..1024");
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);

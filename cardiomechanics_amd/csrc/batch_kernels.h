@@ -44,10 +44,13 @@ int64_t bicg_part_ld(int64_t N);   // partial-sum slots per set the kernels need
 void launch_bicg_init(const BatchBiCG& k, int64_t N, const double* b, const double* rtol, const int32_t* active,
                       hipStream_t s);
 // one BiCGStab iteration of every running set; chunked: 144-byte value chunks (values
-// 16-byte aligned per set and padded, as for launch_spmv)
+// 16-byte aligned per set and padded, as for launch_spmv).  The preconditioner is
+// block Jacobi with dinv, or -- one set only -- apply(apply_ctx, u, z) launching
+// z = P^-1 u on the stream (dinv then null)
+typedef void (*PrecondApply)(void* apply_ctx, const double* u, double* z);
 void launch_bicg_iteration(const BatchBiCG& k, int lanes, bool chunked, int64_t N, const int64_t* rowptr,
                            const int32_t* colind, const void* vals, bool fp32, int64_t set_vals, const double* dinv,
-                           hipStream_t s);
+                           hipStream_t s, PrecondApply apply = nullptr, void* apply_ctx = nullptr);
 
 int batch_dot_blocks(int64_t n);
 int batch_spmv_blocks(int64_t N, int lanes);

@@ -408,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_alpha(BatchBiCG k, int nb) {
     }
 }
 
-// s = r - alpha v, sh = D^-1 s
+// s = r - alpha v, sh = D^-1 s (s only without dinv)
 __global__ __launch_bounds__(kThreads) void k_bicg_s(BatchBiCG k, int64_t N, const double* dinv) {
     const int64_t set = blockIdx.y;
     if (k.done[set]) return;
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_s(BatchBiCG k, int64_t N, con
         sv[c] = k.r[o + c] - al * k.v[o + c];
         k.s[o + c] = sv[c];
     }
-    bj3(dinv + set * 9 * N + 9 * i, sv, k.sh + o);
+    if (dinv) bj3(dinv + set * 9 * N + 9 * i, sv, k.sh + o);
 }
 
 __global__ __launch_bounds__(kThreads) void k_bicg_omega(BatchBiCG k, int nb) {
@@ -532,11 +532,14 @@ void launch_bicg_init(const BatchBiCG& k, int64_t N, const double* b, const doub
 
 void launch_bicg_iteration(const BatchBiCG& k, int lanes, bool chunked, int64_t N, const int64_t* rowptr,
                            const int32_t* colind, const void* vals, bool fp32, int64_t set_vals, const double* dinv,
-                           hipStream_t s) {
+                           hipStream_t s, PrecondApply apply, void* apply_ctx) {
     const int nbn = batch_node_blocks(N);
     const int nbs = batch_spmv_blocks(N, lanes);
     const int64_t n = 3 * N;
-    hipLaunchKernelGGL(k_bicg_precond_p, dim3(nbn, k.S), dim3(kThreads), 0, s, k, N, dinv);
+    if (apply)
+        apply(apply_ctx, k.p, k.ph);
+    else
+        hipLaunchKernelGGL(k_bicg_precond_p, dim3(nbn, k.S), dim3(kThreads), 0, s, k, N, dinv);
 #define CMG_BSPMV(VT, L, D, U, Y, W)                                                                              \
     if (chunked)                                                                                                  \
         hipLaunchKernelGGL((k_bicg_spmv<VT, L, D, true>), dim3(nbs, k.S), dim3(kThreads), 0, s, N, rowptr, colind, \
@@ -556,7 +559,8 @@ void launch_bicg_iteration(const BatchBiCG& k, int lanes, bool chunked, int64_t 
     }
     CMG_BSPMV_L(1, k.ph, k.v, (const double*)nullptr)
     hipLaunchKernelGGL(k_bicg_alpha, dim3(k.S), dim3(kThreads), 0, s, k, nbs);
-    hipLaunchKernelGGL(k_bicg_s, dim3(nbn, k.S), dim3(kThreads), 0, s, k, N, dinv);
+    hipLaunchKernelGGL(k_bicg_s, dim3(nbn, k.S), dim3(kThreads), 0, s, k, N, apply ? nullptr : dinv);
+    if (apply) apply(apply_ctx, k.s, k.sh);
     CMG_BSPMV_L(kBicgDots, k.sh, k.t, k.s)
 #undef CMG_BSPMV_L
 #undef CMG_BSPMV

@@ -134,6 +134,12 @@ struct cmg_ctx {
     int64_t sw_P = 0;
     int sw_nd_max = 0;
     cmg::DevBuf sw_pptr, sw_pnodes, sw_sptr, sw_slots, sw_iptr, sw_cptr, sw_cpos, sw_inv, sw_y;
+    // patches by size class for the setup launches: sw_order[0, sw_n_small) have at most
+    // sw_nd_small DOFs, the rest up to sw_nd_max
+    cmg::DevBuf sw_order;
+    bool precond_built = false;   // the preconditioner holds a factorisation of this Newton solve
+    int64_t sw_n_small = 0;
+    int sw_nd_small = 0;
 };
 
 namespace cmg {

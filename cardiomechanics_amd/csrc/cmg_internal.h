@@ -11,6 +11,19 @@
 
 namespace cmg {
 
+#ifdef __HIPCC__
+// Workgroups are dispatched round-robin over the 8 XCDs; this maps workgroup bid of
+// nblk to a tile index so every XCD gets a contiguous range of tiles and neighbouring
+// tiles share one L2.
+__device__ __forceinline__ int xcd_swizzle(int bid, int nblk) {
+    const int nx = 8;
+    const int full = (nblk / nx) * nx;
+    if (bid >= full) return bid;
+    const int per = nblk / nx;
+    return (bid % nx) * per + bid / nx;
+}
+#endif
+
 // Device copy of one material (same meaning as cmg_material).
 struct DevMaterial {
     int law;

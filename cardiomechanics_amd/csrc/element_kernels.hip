@@ -24,16 +24,6 @@
 namespace cmg {
 namespace {
 
-__device__ __forceinline__ int xcd_swizzle(int bid, int nblk) {
-    // Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
-    // contiguous range of tiles so neighbouring elements share one L2.
-    const int nx = 8;
-    const int full = (nblk / nx) * nx;
-    if (bid >= full) return bid;
-    const int per = nblk / nx;
-    return (bid % nx) * per + bid / nx;
-}
-
 // ---------------------------------------------------------------------------
 // analytic time functions (CBAnalyticFunction.h:53-145), steady Linear window
 // ---------------------------------------------------------------------------
@@ -706,16 +696,28 @@ __global__ __launch_bounds__(64) void k_residual_tangent(const ElemArgs a0) {
 // ---------------------------------------------------------------------------
 constexpr int kQRec = 51;   // Ft(9) BBt(9) S(9) Z(9) Y|X(9) czz codot ceq d00 d11 mfs
 
-template <int NPE, int NQP, typename TT, int NSPLIT, int BSPLIT = 1, int GROUPS = 1, bool LAND17 = false>
+// STAGED: the accumulators of the workgroup's elements go through LDS (the quadrature
+// records' space, free after the contraction) in rounds of RA node rows a, and the
+// workgroup writes each element's rows as one contiguous run of 16-byte stores --
+// instead of every lane storing its 3x3 entries 72 bytes at a time, 7200 bytes apart.
+template <int NPE, int NQP, typename TT, int NSPLIT, int BSPLIT = 1, int GROUPS = 1, bool LAND17 = false,
+          bool STAGED = false>
 __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(const ElemArgs a0) {
     constexpr int EPW = kTanElemsPerWave;
     constexpr int EPG = EPW * GROUPS;     // elements per workgroup
     static_assert(EPG * NQP <= 64, "stage A runs in the first wave");
+    static_assert(!STAGED || NSPLIT == 1, "the staged store keeps every accumulator of the lane");
     ElemArgs a;
     if (!batch_select(a0, a)) return;   // uniform per workgroup: no barrier below is skipped by part of it
-    __shared__ double sT[EPG][NQP][3 * NPE];
-    __shared__ TT sG[EPG][NQP][NPE * 3];
-    __shared__ TT sQ[EPG][NQP][kQRec];
+    constexpr int ROW = 9 * NPE;                                        // K_e values of one node row a
+    constexpr int RA = STAGED ? (sizeof(TT) == 8 ? NPE / 2 : NPE) : 1;   // node rows per staging round
+    constexpr size_t kRecBytes = sizeof(double) * EPG * NQP * 3 * NPE + sizeof(TT) * EPG * NQP * (NPE * 3 + kQRec);
+    constexpr size_t kStageBytes = STAGED ? sizeof(TT) * EPG * RA * ROW : 0;
+    __shared__ __align__(16) unsigned char smem[kRecBytes > kStageBytes ? kRecBytes : kStageBytes];
+    auto& sT = *reinterpret_cast<double (*)[EPG][NQP][3 * NPE]>(smem);
+    auto& sG = *reinterpret_cast<TT (*)[EPG][NQP][NPE * 3]>(smem + sizeof(double) * EPG * NQP * 3 * NPE);
+    auto& sQ = *reinterpret_cast<TT (*)[EPG][NQP][kQRec]>(smem + sizeof(double) * EPG * NQP * 3 * NPE +
+                                                          sizeof(TT) * EPG * NQP * NPE * 3);
     const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x;
     const int64_t E = a.E;
@@ -802,7 +804,13 @@ __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(co
     const int ii = ij / 3, jj = ij - 3 * (ij / 3);
     const int slot = grp * EPW + lslot;
     const int64_t e = e0 + slot;
-    if (lslot >= EPW || e >= E) return;
+    const bool valid = lslot < EPW && e < E;
+    if (!STAGED && !valid) return;
+    constexpr int AH = NPE / NSPLIT;
+    constexpr int BH = NPE / BSPLIT;
+    const int b0 = bh * BH;
+    TT keep[STAGED ? AH * BH : 1];   // V K_e entries of this lane for the staged store
+    if (valid) {
     const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
     const bool guc = m.law == CMG_LAW_GUCCIONE;
     TT bmat[9];
@@ -814,9 +822,6 @@ __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(co
     }
     const double V = a.w[e];
     TT* Ke = static_cast<TT*>(a.Ke) + e * (9 * NPE * NPE);
-    constexpr int AH = NPE / NSPLIT;
-    constexpr int BH = NPE / BSPLIT;
-    const int b0 = bh * BH;
 #pragma unroll 1
     for (int half = 0; half < NSPLIT; half++) {
         TT acc[AH * BH];
@@ -907,11 +912,16 @@ __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(co
                 contract(std::integral_constant<int, BH>());
         }
         const TT Vt = (TT)V;
+        if constexpr (STAGED) {
 #pragma unroll
-        for (int aa = 0; aa < AH; aa++)
+            for (int r = 0; r < AH * BH; r++) keep[r] = Vt * acc[r];
+        } else {
 #pragma unroll
-            for (int bb = 0; bb < BH; bb++)
-                Ke[9 * ((half * AH + aa) * NPE + b0 + bb) + ij] = Vt * acc[aa * BH + bb];
+            for (int aa = 0; aa < AH; aa++)
+#pragma unroll
+                for (int bb = 0; bb < BH; bb++)
+                    Ke[9 * ((half * AH + aa) * NPE + b0 + bb) + ij] = Vt * acc[aa * BH + bb];
+        }
     }
     if (jj == 0 && bh == 0) {
         double* fe = a.fe + e * (3 * NPE);
@@ -921,6 +931,36 @@ __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(co
 #pragma unroll
             for (int q = 0; q < NQP; q++) t += sT[slot][q][3 * k + ii];
             fe[3 * k + ii] = V * t;
+        }
+    }
+    }   // valid
+    if constexpr (STAGED) {
+        // 16-byte units: RA rows of one element are RA * ROW values, a whole number of units,
+        // and every element's and round's start is 16-byte aligned (ROW * sizeof(TT) * RA and
+        // 9 * NPE * NPE * sizeof(TT) are multiples of 16 for the shapes instantiated)
+        constexpr int VPU = 16 / sizeof(TT);
+        constexpr int UNITS = RA * ROW / VPU;
+        static_assert((RA * ROW) % VPU == 0 && (RA * ROW * sizeof(TT)) % 16 == 0 &&
+                          (9 * NPE * NPE * sizeof(TT)) % 16 == 0, "16-byte runs");
+        constexpr int NT = 64 * BSPLIT * GROUPS;
+        auto& stage = *reinterpret_cast<TT (*)[EPG][RA][ROW]>(smem);
+        const int nel = (int)((E - e0 < EPG) ? (E - e0) : EPG);
+#pragma unroll
+        for (int r0 = 0; r0 < NPE; r0 += RA) {
+            __syncthreads();   // the records (first round) or the previous round are consumed
+            if (valid) {
+#pragma unroll
+                for (int aa = 0; aa < RA; aa++)
+#pragma unroll
+                    for (int bb = 0; bb < BH; bb++) stage[slot][aa][9 * (b0 + bb) + ij] = keep[(r0 + aa) * BH + bb];
+            }
+            __syncthreads();
+            for (int u = lane; u < nel * UNITS; u += NT) {
+                const int sl = u / UNITS, off = u - sl * UNITS;
+                const uint4 v = reinterpret_cast<const uint4*>(&stage[sl][0][0])[off];
+                TT* dst = static_cast<TT*>(a.Ke) + (e0 + sl) * (9 * NPE * NPE) + r0 * ROW;
+                reinterpret_cast<uint4*>(dst)[off] = v;
+            }
         }
     }
 }
@@ -1268,6 +1308,12 @@ void launch_element_residual_tangent(int npe, bool fp32, const ElemArgs& a, hipS
         } else if (variant == 6) {
             if (fp32) CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, float, 1, 2, 2>), 256);
             else CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, double, 1, 2, 2>), 256);
+        } else if (variant == 7) {
+            if (fp32) CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, float, 1, 2, 2, false, true>), 256);
+            else CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, double, 1, 2, 2, false, true>), 256);
+        } else if (variant == 8) {
+            if (fp32) CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, float, 1, 1, 2, false, true>), 128);
+            else CMG_LAUNCH_G2((k_residual_tangent_v2<10, 4, double, 1, 1, 2, false, true>), 128);
         } else if (variant == 1) {
             if (fp32) CMG_LAUNCH((k_residual_tangent<10, 4, float>));
             else CMG_LAUNCH((k_residual_tangent<10, 4, double>));

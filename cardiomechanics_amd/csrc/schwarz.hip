@@ -28,25 +28,36 @@ namespace {
 
 constexpr int kWave = 64;
 
-// A_p^{-1} of every patch.  Dynamic LDS: nd_max^2 + nd_max doubles + nd_max ints.
+// A_p^{-1} of the patches order[0, n), one wave per patch.  Dynamic LDS: nd_cap^2 +
+// nd_cap doubles + nd_cap ints for the class's largest patch.  Gauss-Jordan with
+// partial pivoting; in the elimination each lane owns columns j = lane, lane + 64 and
+// walks the rows, so no index is divided and the pivot row stays in registers.
 template <typename VT, typename IT>
-__global__ __launch_bounds__(kWave) void k_patch_setup(int64_t P, const int32_t* pptr, const int32_t* pnodes,
+__global__ __launch_bounds__(kWave) void k_patch_setup(int64_t n, const int32_t* order, const int32_t* pptr,
                                                        const int64_t* sptr, const int32_t* slots, const int64_t* iptr,
-                                                       const VT* vals, IT* inv, int nd_max) {
+                                                       const VT* vals, IT* inv, int nd_cap) {
     extern __shared__ double lds[];
-    const int64_t p = blockIdx.x;
-    if (p >= P) return;
+    if ((int64_t)blockIdx.x >= n) return;
+    const int64_t p = order[blockIdx.x];
     const int lane = threadIdx.x;
     const int np = pptr[p + 1] - pptr[p];
     const int nd = 3 * np;
-    double* A = lds;
-    double* col = lds + (size_t)nd_max * nd_max;
-    int* perm = reinterpret_cast<int*>(col + nd_max);
+    double* A = lds;                                   // row-major nd x nd
+    double* col = lds + (size_t)nd_cap * nd_cap;
+    int* perm = reinterpret_cast<int*>(col + nd_cap);
     const int32_t* sl = slots + sptr[p];
-    for (int e = lane; e < nd * nd; e += kWave) {
-        const int I = e / nd, J = e - (e / nd) * nd;
-        const int s = sl[(I / 3) * np + J / 3];
-        A[e] = s >= 0 ? (double)vals[9 * (int64_t)s + 3 * (I % 3) + (J % 3)] : 0.0;
+    const int j0 = lane, j1 = lane + kWave;            // this lane's columns (j1 only when nd > 64)
+    const int bj0 = j0 / 3, cj0 = j0 - 3 * bj0, bj1 = j1 / 3, cj1 = j1 - 3 * bj1;
+    for (int I = 0; I < nd; I++) {
+        const int bi = I / 3, ci = I - 3 * bi;
+        if (j0 < nd) {
+            const int s = sl[bi * np + bj0];
+            A[I * nd + j0] = s >= 0 ? (double)vals[9 * (int64_t)s + 3 * ci + cj0] : 0.0;
+        }
+        if (j1 < nd) {
+            const int s = sl[bi * np + bj1];
+            A[I * nd + j1] = s >= 0 ? (double)vals[9 * (int64_t)s + 3 * ci + cj1] : 0.0;
+        }
     }
     __syncthreads();
     for (int k = 0; k < nd; k++) {
@@ -70,28 +81,37 @@ __global__ __launch_bounds__(kWave) void k_patch_setup(int64_t P, const int32_t*
             }
         }
         if (lane == 0) perm[k] = bi;
-        if (bi != k)
-            for (int j = lane; j < nd; j += kWave) {
-                const double t = A[k * nd + j];
-                A[k * nd + j] = A[bi * nd + j];
-                A[bi * nd + j] = t;
-            }
-        __syncthreads();
-        const double piv = A[k * nd + k];
+        // swap rows k and bi, scale row k by 1 / pivot (the pivot entry becomes 1 / pivot);
+        // the scaled pivot row stays in registers
+        double rk0 = 0.0, rk1 = 0.0, a0 = 0.0, a1 = 0.0;
+        if (j0 < nd) {
+            a0 = A[bi * nd + j0];
+            rk0 = A[k * nd + j0];
+        }
+        if (j1 < nd) {
+            a1 = A[bi * nd + j1];
+            rk1 = A[k * nd + j1];
+        }
+        const double piv = __shfl(k < kWave ? a0 : a1, k & (kWave - 1));   // A[bi][k]
         const double ip = 1.0 / piv;
         __syncthreads();
-        for (int j = lane; j < nd; j += kWave) A[k * nd + j] = (j == k) ? ip : A[k * nd + j] * ip;
+        if (bi != k) {
+            if (j0 < nd) A[bi * nd + j0] = rk0;
+            if (j1 < nd) A[bi * nd + j1] = rk1;
+        }
+        rk0 = (j0 == k) ? ip : a0 * ip;
+        rk1 = (j1 == k) ? ip : a1 * ip;
+        if (j0 < nd) A[k * nd + j0] = rk0;
+        if (j1 < nd) A[k * nd + j1] = rk1;
+        __syncthreads();
         // column k of the other rows, read before any of them changes
         for (int i = lane; i < nd; i += kWave) col[i] = A[i * nd + k];
         __syncthreads();
-        for (int e = lane; e < nd * nd; e += kWave) {
-            const int i = e / nd, j = e - (e / nd) * nd;
+        for (int i = 0; i < nd; i++) {
             if (i == k) continue;
             const double f = col[i];
-            if (j == k)
-                A[e] = -f * ip;
-            else
-                A[e] -= f * A[k * nd + j];
+            if (j0 < nd) A[i * nd + j0] = (j0 == k) ? -f * ip : A[i * nd + j0] - f * rk0;
+            if (j1 < nd) A[i * nd + j1] = (j1 == k) ? -f * ip : A[i * nd + j1] - f * rk1;
         }
         __syncthreads();
     }
@@ -107,10 +127,8 @@ __global__ __launch_bounds__(kWave) void k_patch_setup(int64_t P, const int32_t*
         __syncthreads();
     }
     IT* out = inv + iptr[p];
-    for (int e = lane; e < nd * nd; e += kWave) {
-        const int j = e / nd, i = e - (e / nd) * nd;   // column-major: out[j nd + i] = Ainv[i][j]
-        out[e] = (IT)A[i * nd + j];
-    }
+    for (int j = 0; j < nd; j++)   // column-major: out[j nd + i] = Ainv[i][j]
+        for (int i = lane; i < nd; i += kWave) out[(int64_t)j * nd + i] = (IT)A[i * nd + j];
 }
 
 // y_p = A_p^{-1} r_p, one wave per patch, four patches per workgroup
@@ -184,6 +202,16 @@ int build_vertex_patches(cmg_ctx* c, const int32_t* elements) {
         np_max = std::max(np_max, (int)q.size());
     }
     if (3 * np_max > kMaxPatchDofs) return CMG_SUCCESS;   // stays with block Jacobi
+    // setup launches by size class: patches of at most kSmallPatchDofs DOFs (most of
+    // them) with that much LDS per wave, the rest with room for the largest
+    constexpr int kSmallPatchDofs = 48;
+    std::vector<int32_t> order;
+    order.reserve(P);
+    for (int64_t p = 0; p < P; p++)
+        if (3 * (int)patch[p].size() <= kSmallPatchDofs) order.push_back((int32_t)p);
+    const int64_t n_small = (int64_t)order.size();
+    for (int64_t p = 0; p < P; p++)
+        if (3 * (int)patch[p].size() > kSmallPatchDofs) order.push_back((int32_t)p);
     std::vector<int32_t> pptr(P + 1, 0), pnodes;
     std::vector<int64_t> sptr(P + 1, 0), iptr(P + 1, 0);
     std::vector<int32_t> slots;
@@ -218,27 +246,36 @@ int build_vertex_patches(cmg_ctx* c, const int32_t* elements) {
     CMG_TRY(upload(c->sw_iptr, iptr.data(), iptr.size(), s));
     CMG_TRY(upload(c->sw_cptr, cnt.data(), cnt.size(), s));
     CMG_TRY(upload(c->sw_cpos, cpos.data(), cpos.size(), s));
+    CMG_TRY(upload(c->sw_order, order.data(), order.size(), s));
     CMG_TRY(alloc(c->sw_inv, (size_t)iptr[P] * (c->fp32 ? sizeof(float) : sizeof(double))));
     CMG_TRY(alloc(c->sw_y, (size_t)3 * pnodes.size() * sizeof(double)));
     CMG_HIP_CHECK(hipStreamSynchronize(s));
     c->sw_P = P;
     c->sw_nd_max = 3 * np_max;
+    c->sw_n_small = n_small;
+    c->sw_nd_small = std::min(kSmallPatchDofs, 3 * np_max);
     c->schwarz = true;
     return CMG_SUCCESS;
 }
 
 void launch_patch_setup(cmg_ctx* c, const void* vals) {
-    const int ndm = c->sw_nd_max;
-    const size_t lds = (size_t)(ndm * ndm + ndm) * sizeof(double) + (size_t)ndm * sizeof(int);
-#define CMG_PSETUP(VT, IT)                                                                                       \
-    hipLaunchKernelGGL((k_patch_setup<VT, IT>), dim3((unsigned)c->sw_P), dim3(kWave), lds, c->stream, c->sw_P,  \
-                       c->sw_pptr.as<int32_t>(), c->sw_pnodes.as<int32_t>(), c->sw_sptr.as<int64_t>(),           \
-                       c->sw_slots.as<int32_t>(), c->sw_iptr.as<int64_t>(), (const VT*)vals, c->sw_inv.as<IT>(), ndm)
-    if (c->fp32)
-        CMG_PSETUP(float, float);
-    else
-        CMG_PSETUP(double, double);
+    const int64_t cls_n[2] = {c->sw_n_small, c->sw_P - c->sw_n_small};
+    const int cls_nd[2] = {c->sw_nd_small, c->sw_nd_max};
+    for (int k = 0; k < 2; k++) {
+        if (cls_n[k] == 0) continue;
+        const int nd = cls_nd[k];
+        const size_t lds = (size_t)(nd * nd + nd) * sizeof(double) + (size_t)nd * sizeof(int);
+        const int32_t* order = c->sw_order.as<int32_t>() + (k ? c->sw_n_small : 0);
+#define CMG_PSETUP(VT, IT)                                                                                        \
+    hipLaunchKernelGGL((k_patch_setup<VT, IT>), dim3((unsigned)cls_n[k]), dim3(kWave), lds, c->stream, cls_n[k], \
+                       order, c->sw_pptr.as<int32_t>(), c->sw_sptr.as<int64_t>(), c->sw_slots.as<int32_t>(),      \
+                       c->sw_iptr.as<int64_t>(), (const VT*)vals, c->sw_inv.as<IT>(), nd)
+        if (c->fp32)
+            CMG_PSETUP(float, float);
+        else
+            CMG_PSETUP(double, double);
 #undef CMG_PSETUP
+    }
 }
 
 void launch_patch_apply(cmg_ctx* c, const double* r, double* z) {

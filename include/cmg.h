@@ -123,12 +123,13 @@ typedef struct cmg_options {
     int32_t element_variant;    /* residual+tangent kernel: 2 shared quadrature stage, 3 the same in
                                    two node passes, 4 the same with two lanes per element and
                                    spatial pair, 1 per-lane quadrature points, 5 / 6 variants 2 / 4
-                                   with 14 elements per workgroup; 0 (default): 6 in FP64, 5 in
-                                   mixed precision.  A context with a Land17 material always
-                                   runs the shape of variant 6 */
+                                   with 14 elements per workgroup, 7 / 8 variants 6 / 5 with the
+                                   element tangents written through LDS as contiguous 16-byte
+                                   runs; 0 (default): 6 in FP64, 5 in mixed precision.  A context
+                                   with a Land17 material always runs the shape of variant 6 */
     int32_t krylov_method;      /* 0 GMRES(restart), 1 BiCGStab with device-side scalars and graph-replayed
                                    iterations, 2 BiCGStab with host-side scalars (default); all right
-                                   block-Jacobi preconditioned */
+                                   preconditioned (see preconditioner) */
     double newton_atol;         /* Solver.Precision used as SNES abstol/rtol/stol (default 1e-8) */
     double newton_rtol;
     double newton_stol;
@@ -149,9 +150,12 @@ typedef struct cmg_options {
     int32_t preconditioner;     /* of the Krylov solves: 1 3x3 nodal block Jacobi, 2 vertex-patch
                                    additive Schwarz (T10: each element vertex with the midside
                                    nodes of its edges, patch matrices inverted on the device;
-                                   block Jacobi for T4 and for krylov_method 1); 0 (default):
+                                   block Jacobi for T4); 0 (default):
                                    2 where it applies */
-    int32_t reserved_i;
+    int32_t precond_reuse;      /* 1 (default): the preconditioner is built from the first tangent
+                                   of each Newton solve and reused by its later linear solves (the
+                                   reference's SNESSetLagPreconditioner(-2), CBSolverNewmarkBeta.cpp:
+                                   173-174); 0: built from every tangent */
     double krylov_eta_max;      /* upper bound of the Eisenstat-Walker forcing term of the Newton
                                    linear solves (default 0.5; 0 reads as the default) */
 } cmg_options;
@@ -356,7 +360,7 @@ typedef struct cmg_batch_info {
     int32_t use_graph;
     int32_t tuned;             /* 1 once the autotuner has run */
     int32_t reserved[2];
-    double element_ms[6];      /* measured launch time of element variants 1..6 (0: not timed) */
+    double element_ms[8];      /* measured launch time of element variants 1..8 (0: not timed) */
     double spmv_ms[3];         /* measured batched SpMV time at 4, 8, 16 lanes */
     double tune_seconds;
 } cmg_batch_info;
@@ -367,7 +371,7 @@ typedef struct cmg_batch_info {
  * outlive it.  Every set starts from ctx's current node coordinates. */
 int cmg_batch_create(cmg_ctx* ctx, int32_t n_sets, const cmg_material* materials, cmg_batch** out);
 int cmg_batch_destroy(cmg_batch* batch);
-/* use_graph 0/1; spmv_lanes 0 (autotune) or 4/8/16; element_variant 0 (autotune) or 1..6;
+/* use_graph 0/1; spmv_lanes 0 (autotune) or 4/8/16; element_variant 0 (autotune) or 1..8;
  * cg_chunk >= 1.  Call before the first step. */
 int cmg_batch_configure(cmg_batch* batch, int32_t use_graph, int32_t spmv_lanes, int32_t element_variant,
                         int32_t cg_chunk);

@@ -35,7 +35,7 @@ def test_struct_layouts_match_header_sizes():
     assert ctypes.sizeof(_lib.CmgMesh) == 3 * 8 + 4 * 4 + 8 * 8
     assert ctypes.sizeof(_lib.CmgOptions) == 10 * 4 + 7 * 8 + 4 * 8
     assert ctypes.sizeof(_lib.CmgStepStats) == 4 * 4 + 6 * 8 + 2 * 4 + 8
-    assert ctypes.sizeof(_lib.CmgBatchInfo) == 8 * 4 + 10 * 8
+    assert ctypes.sizeof(_lib.CmgBatchInfo) == 8 * 4 + 12 * 8
 
 
 def test_default_options_are_the_reference_defaults():

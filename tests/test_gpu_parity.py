@@ -103,7 +103,7 @@ def test_element_tangents_match_complex_step(case):
 
 @pytest.mark.parametrize("tension", ["sinus", "tanh"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
-@pytest.mark.parametrize("variant", [1, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 3, 4, 5, 6, 7, 8])
 def test_element_kernel_variants_agree_with_the_oracle(variant, precision, tension):
     """every residual+tangent kernel variant against the complex-step oracle (and the
     residual against the oracle residual), T10 LV with Holzapfel and active tension
@@ -215,8 +215,7 @@ def test_t4_static_load_steps_match_oracle():
 @pytest.mark.parametrize("method", [0, 1, 2], ids=["gmres", "bicgstab_graph", "bicgstab"])
 def test_spmv_and_krylov_solve(method, pc):
     """The Newmark system matrix K + M/(beta dt^2) at the unloaded state, dt = 1 ms;
-    block Jacobi or vertex-patch Schwarz preconditioning (the device-scalar BiCGStab
-    always uses block Jacobi)."""
+    block Jacobi or vertex-patch Schwarz preconditioning."""
     mesh, mats = lv_small()
     rng = np.random.default_rng(3)
     with MechanicsContext(mesh, mats, krylov_restart=60, krylov_maxit=20000, krylov_method=method,
@@ -302,16 +301,19 @@ def test_static_mixed_precision_converges_to_fp64_root():
         assert err <= 1e-8
 
 
-@pytest.mark.parametrize("method,pc", [(0, 1), (1, 1), (2, 1), (0, 2), (2, 2)],
-                         ids=["gmres", "bicgstab_graph", "bicgstab", "gmres_schwarz", "bicgstab_schwarz"])
-def test_newmark_steps_match_oracle(method, pc):
-    """Newmark (beta=1/4, gamma=1/2) with Holzapfel + active tension + cavity pressure."""
+@pytest.mark.parametrize("method,pc,reuse", [(0, 1, 0), (1, 1, 0), (2, 1, 0), (0, 2, 0), (1, 2, 0), (2, 2, 0),
+                                             (2, 2, 1)],
+                         ids=["gmres", "bicgstab_graph", "bicgstab", "gmres_schwarz", "bicgstab_graph_schwarz",
+                              "bicgstab_schwarz", "bicgstab_schwarz_reused"])
+def test_newmark_steps_match_oracle(method, pc, reuse):
+    """Newmark (beta=1/4, gamma=1/2) with Holzapfel + active tension + cavity pressure;
+    reuse: the preconditioner of each step's first tangent serves all its linear solves."""
     mesh, mats = lv_small()
     cpu = CpuMechanics(mesh, mats)
     plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
     dt = 0.005
     with MechanicsContext(mesh, mats, newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15,
-                          krylov_method=method, preconditioner=pc) as ctx:
+                          krylov_method=method, preconditioner=pc, precond_reuse=reuse) as ctx:
         for k in range(1, 9):
             t = k * dt
             p = plug.apply(t, dt)[0]
