@@ -5,8 +5,11 @@
 // every kernel launch covers all sets at once: element kernels and assembly with
 // blockIdx.y = set, and a block-Jacobi conjugate gradient whose scalar recurrences
 // live on the device, so a run of `cg_chunk` iterations is one HIP graph replay
-// followed by one small copy of the convergence flags.  An autotuner times the
-// element-kernel variants and the SpMV lane count on the first tangent.
+// followed by one small copy of the convergence flags.  The conjugate gradient is
+// preconditioned by the context's vertex-patch additive Schwarz (one set of patch
+// inverses per parameter set; symmetric like the tangents), or by 3x3 block Jacobi
+// where the context has no patches.  An autotuner times the element-kernel variants
+// and the SpMV lane count on the first tangent.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -30,6 +33,9 @@ struct cmg_batch {
     DevBuf fe, Ke, fs, Ks, vol;                    // per set element / surface outputs
     bool ptan = false;                             // surface-pressure tangent in the batched matrices
     DevBuf vals, dinv;                             // [S][nnzb*9], [S][N*9]
+    DevBuf sw_inv, sw_y;                           // Schwarz: [S][sw_inv_len], [S][sw_y_len]
+    bool schwarz = false;
+    const int32_t* apply_skip = nullptr;           // sets the Schwarz application skips
     DevBuf cg_r, cg_z, cg_p, cg_q;                 // [S][n]
     DevBuf sc;                                     // CG scalars: 6 doubles per set
     DevBuf flags;                                  // done[S], it[S]
@@ -171,6 +177,16 @@ void bgather_tangent(cmg_batch* b) {
     st.active = b->active.as<int32_t>();
     launch_gather_matrix(gather_maps(c), b->Ke.p, c->fp32, b->ptan ? b->Ks.as<double>() : c->Ks.as<double>(), nullptr,
                          0.0, c->bcmask.as<uint8_t>(), b->vals.p, c->fp32, false, s, st);
+    if (b->schwarz) {
+        PatchSets ps;
+        ps.S = b->S;
+        ps.set_vals = c->nnzb * 9;
+        ps.set_inv = c->sw_inv_len;
+        ps.inv = b->sw_inv.p;
+        ps.flags = b->active.as<int32_t>();
+        launch_patch_setup(c, b->vals.p, ps);
+        return;
+    }
     SetStrides sj;
     sj.n_sets = b->S;
     sj.a = c->nnzb * 9;
@@ -178,6 +194,20 @@ void bgather_tangent(cmg_batch* b) {
     sj.active = b->active.as<int32_t>();
     launch_block_jacobi_setup(c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(), b->vals.p, c->fp32,
                               b->dinv.as<double>(), s, sj);
+}
+
+// z = P^-1 u of every set not in b->apply_skip (the Schwarz application of the CG)
+void batch_schwarz_apply(void* bb, const double* u, double* z) {
+    cmg_batch* b = static_cast<cmg_batch*>(bb);
+    PatchSets ps;
+    ps.S = b->S;
+    ps.set_inv = b->c->sw_inv_len;
+    ps.set_vec = b->n;
+    ps.set_y = b->c->sw_y_len;
+    ps.inv = b->sw_inv.p;
+    ps.y = b->sw_y.as<double>();
+    ps.flags = b->apply_skip;
+    launch_patch_apply(b->c, u, z, ps);
 }
 
 // out[k*S + s] = x_k,s . y_k,s for the given pairs; one synchronisation
@@ -228,7 +258,8 @@ void cg_chunk_eager(cmg_batch* b) {
     cmg_ctx* c = b->c;
     for (int k = 0; k < b->info.cg_chunk; k++)
         launch_bcg_iteration(b->cg, b->info.spmv_lanes, c->N, c->rowptr.as<int64_t>(), c->colind.as<int32_t>(),
-                             b->vals.p, c->fp32, c->nnzb * 9, b->dinv.as<double>(), c->stream);
+                             b->vals.p, c->fp32, c->nnzb * 9, b->dinv.as<double>(), c->stream,
+                             b->schwarz ? batch_schwarz_apply : nullptr, b);
 }
 
 int build_graph(cmg_batch* b) {
@@ -250,8 +281,10 @@ int bcg_solve(cmg_batch* b, const std::vector<double>& rtol, std::vector<int32_t
     const int S = b->S;
     std::memcpy(b->h_dbl, rtol.data(), S * sizeof(double));
     CMG_HIP_CHECK(hipMemcpyAsync(b->rtol.p, b->h_dbl, S * sizeof(double), hipMemcpyHostToDevice, s));
+    b->apply_skip = nullptr;   // init: every set (the inactive ones' results are not used)
     launch_bcg_init(b->cg, c->N, b->rhs.as<double>(), b->dinv.as<double>(), b->rtol.as<double>(),
-                    b->active.as<int32_t>(), s);
+                    b->active.as<int32_t>(), s, b->schwarz ? batch_schwarz_apply : nullptr, b);
+    b->apply_skip = b->cg.done;   // iterations: the sets still running
     if (!b->lanes_fixed) CMG_TRY(tune_spmv(b));
     if (b->info.use_graph && !b->graph) CMG_TRY(build_graph(b));
     b->info.tuned = 1;
@@ -461,6 +494,11 @@ int cmg_batch_create(cmg_ctx* c, int32_t n_sets, const cmg_material* materials, 
     }
     CMG_TRY(alloc(b->vals, (size_t)S * c->nnzb * 9 * kt));
     CMG_TRY(alloc(b->dinv, (size_t)S * c->N * 9 * sizeof(double)));
+    if (c->schwarz) {
+        CMG_TRY(alloc(b->sw_inv, (size_t)S * c->sw_inv_len * kt));
+        CMG_TRY(alloc(b->sw_y, (size_t)S * c->sw_y_len * sizeof(double)));
+        b->schwarz = true;
+    }
     CMG_TRY(alloc(b->sc, (size_t)S * 6 * sizeof(double)));
     CMG_TRY(alloc(b->flags, (size_t)S * 2 * sizeof(int32_t)));
     b->part_ld = std::max<int64_t>({(int64_t)batch_spmv_blocks(c->N, 16), 2 * (int64_t)batch_node_blocks(c->N),

@@ -14,6 +14,9 @@ struct SetCoefs {
     double c[kMaxSets];
 };
 
+// a preconditioner application z = P^-1 u launched on the caller's stream
+typedef void (*PrecondApply)(void* apply_ctx, const double* u, double* z);
+
 // Device state of S preconditioned conjugate-gradient solves (all arrays device memory).
 struct BatchCG {
     int S = 0;
@@ -47,7 +50,6 @@ void launch_bicg_init(const BatchBiCG& k, int64_t N, const double* b, const doub
 // 16-byte aligned per set and padded, as for launch_spmv).  The preconditioner is
 // block Jacobi with dinv, or -- one set only -- apply(apply_ctx, u, z) launching
 // z = P^-1 u on the stream (dinv then null)
-typedef void (*PrecondApply)(void* apply_ctx, const double* u, double* z);
 void launch_bicg_iteration(const BatchBiCG& k, int lanes, bool chunked, int64_t N, const int64_t* rowptr,
                            const int32_t* colind, const void* vals, bool fp32, int64_t set_vals, const double* dinv,
                            hipStream_t s, PrecondApply apply = nullptr, void* apply_ctx = nullptr);
@@ -57,11 +59,13 @@ int batch_spmv_blocks(int64_t N, int lanes);
 int batch_node_blocks(int64_t N);
 
 // x = 0, r = b, z = p = D^-1 b and the tolerances ||r|| <= rtol[s] ||b_s|| for the active sets
+// apply: z = P^-1 u for all sets (u, z [S][n]) in place of the block Jacobi with dinv
 void launch_bcg_init(const BatchCG& cg, int64_t N, const double* b, const double* dinv, const double* rtol,
-                     const int32_t* active, hipStream_t s);
+                     const int32_t* active, hipStream_t s, PrecondApply apply = nullptr, void* apply_ctx = nullptr);
 // one CG iteration of every running set (5 launches, no host synchronisation)
 void launch_bcg_iteration(const BatchCG& cg, int lanes, int64_t N, const int64_t* rowptr, const int32_t* colind,
-                          const void* vals, bool fp32, int64_t set_vals, const double* dinv, hipStream_t s);
+                          const void* vals, bool fp32, int64_t set_vals, const double* dinv, hipStream_t s,
+                          PrecondApply apply = nullptr, void* apply_ctx = nullptr);
 // only the q = A p (+ p.q partials) step, for timing the lane count
 void launch_bcg_spmv_dot(const BatchCG& cg, int lanes, int64_t N, const int64_t* rowptr, const int32_t* colind,
                          const void* vals, bool fp32, int64_t set_vals, hipStream_t s);

@@ -87,7 +87,8 @@ __global__ __launch_bounds__(kThreads) void k_bcg_alpha(BatchCG cg, int nb) {
     }
 }
 
-// x += alpha p, r -= alpha q, z = D^-1 r; partial r.z and r.r (one node per thread)
+// x += alpha p, r -= alpha q, z = D^-1 r; partial r.z and r.r (one node per thread).
+// Without dinv only x and r (z then comes from the preconditioner, the partials from k_bcg_rz).
 __global__ __launch_bounds__(kThreads) void k_bcg_update(BatchCG cg, int64_t N, const double* dinv, int nb) {
     __shared__ double sh[4];
     const int64_t set = blockIdx.y;
@@ -95,6 +96,20 @@ __global__ __launch_bounds__(kThreads) void k_bcg_update(BatchCG cg, int64_t N, 
     const int64_t n = 3 * N;
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     const double al = cg.alpha[set];
+    if (!dinv) {
+        if (i < N) {
+            double* x = cg.x + set * n + 3 * i;
+            double* r = cg.r + set * n + 3 * i;
+            const double* p = cg.p + set * n + 3 * i;
+            const double* q = cg.q + set * n + 3 * i;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                x[c] += al * p[c];
+                r[c] = r[c] - al * q[c];
+            }
+        }
+        return;
+    }
     double rz = 0.0, rr = 0.0;
     if (i < N) {
         double* x = cg.x + set * n + 3 * i;
@@ -116,6 +131,33 @@ __global__ __launch_bounds__(kThreads) void k_bcg_update(BatchCG cg, int64_t N, 
             z[c] = zc;
             rz += rv[c] * zc;
             rr += rv[c] * rv[c];
+        }
+    }
+    const double trz = block_sum256(rz, sh);
+    const double trr = block_sum256(rr, sh);
+    if (threadIdx.x == 0) {
+        double* part = cg.part + set * cg.part_ld + 2 * blockIdx.x;
+        part[0] = trz;
+        part[1] = trr;
+    }
+}
+
+// partial r.z and r.r after the preconditioner wrote z (p = z as well when init)
+__global__ __launch_bounds__(kThreads) void k_bcg_rz(BatchCG cg, int64_t N, const int32_t* active, bool init) {
+    __shared__ double sh[4];
+    const int64_t set = blockIdx.y;
+    if (init ? !active[set] : cg.done[set] != 0) return;
+    const int64_t n = 3 * N;
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (i < N) {
+        const int64_t o = set * n + 3 * i;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double rv = cg.r[o + c], zc = cg.z[o + c];
+            if (init) cg.p[o + c] = zc;
+            rz += rv * zc;
+            rr += rv * rv;
         }
     }
     const double trz = block_sum256(rz, sh);
@@ -156,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void k_bcg_p(BatchCG cg, int64_t n) {
     p[i] = cg.z[set * n + i] + b * p[i];
 }
 
-// start: x = 0, r = b, z = D^-1 r, p = z; partial r.z and r.r
+// start: x = 0, r = b, z = D^-1 r, p = z; partial r.z and r.r (without dinv: x and r only)
 __global__ __launch_bounds__(kThreads) void k_bcg_init(BatchCG cg, int64_t N, const double* b, const double* dinv,
                                                        const int32_t* active) {
     __shared__ double sh[4];
@@ -164,6 +206,14 @@ __global__ __launch_bounds__(kThreads) void k_bcg_init(BatchCG cg, int64_t N, co
     if (!active[set]) return;
     const int64_t n = 3 * N;
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (!dinv) {
+        if (i < N)
+            for (int c = 0; c < 3; c++) {
+                cg.x[set * n + 3 * i + c] = 0.0;
+                cg.r[set * n + 3 * i + c] = b[set * n + 3 * i + c];
+            }
+        return;
+    }
     double rz = 0.0, rr = 0.0;
     if (i < N) {
         const int64_t o = set * n + 3 * i;
@@ -601,20 +651,29 @@ void launch_bcg_spmv_dot(const BatchCG& cg, int lanes, int64_t N, const int64_t*
 
 // The spmv writes one partial per block at stride part_ld: part_ld >= blocks (checked by the caller).
 void launch_bcg_iteration(const BatchCG& cg, int lanes, int64_t N, const int64_t* rowptr, const int32_t* colind,
-                          const void* vals, bool fp32, int64_t set_vals, const double* dinv, hipStream_t s) {
+                          const void* vals, bool fp32, int64_t set_vals, const double* dinv, hipStream_t s,
+                          PrecondApply apply, void* apply_ctx) {
     launch_bcg_spmv_dot(cg, lanes, N, rowptr, colind, vals, fp32, set_vals, s);
     hipLaunchKernelGGL(k_bcg_alpha, dim3(cg.S), dim3(kThreads), 0, s, cg, batch_spmv_blocks(N, lanes));
     const int nbn = batch_node_blocks(N);
-    hipLaunchKernelGGL(k_bcg_update, dim3(nbn, cg.S), dim3(kThreads), 0, s, cg, N, dinv, nbn);
+    hipLaunchKernelGGL(k_bcg_update, dim3(nbn, cg.S), dim3(kThreads), 0, s, cg, N, apply ? nullptr : dinv, nbn);
+    if (apply) {
+        apply(apply_ctx, cg.r, cg.z);
+        hipLaunchKernelGGL(k_bcg_rz, dim3(nbn, cg.S), dim3(kThreads), 0, s, cg, N, (const int32_t*)nullptr, false);
+    }
     hipLaunchKernelGGL(k_bcg_beta, dim3(cg.S), dim3(kThreads), 0, s, cg, nbn);
     const int64_t n = 3 * N;
     hipLaunchKernelGGL(k_bcg_p, dim3((unsigned)((n + kThreads - 1) / kThreads), cg.S), dim3(kThreads), 0, s, cg, n);
 }
 
 void launch_bcg_init(const BatchCG& cg, int64_t N, const double* b, const double* dinv, const double* rtol,
-                     const int32_t* active, hipStream_t s) {
+                     const int32_t* active, hipStream_t s, PrecondApply apply, void* apply_ctx) {
     const int nbn = batch_node_blocks(N);
-    hipLaunchKernelGGL(k_bcg_init, dim3(nbn, cg.S), dim3(kThreads), 0, s, cg, N, b, dinv, active);
+    hipLaunchKernelGGL(k_bcg_init, dim3(nbn, cg.S), dim3(kThreads), 0, s, cg, N, b, apply ? nullptr : dinv, active);
+    if (apply) {
+        apply(apply_ctx, cg.r, cg.z);
+        hipLaunchKernelGGL(k_bcg_rz, dim3(nbn, cg.S), dim3(kThreads), 0, s, cg, N, active, true);
+    }
     hipLaunchKernelGGL(k_bcg_init_reduce, dim3(cg.S), dim3(kThreads), 0, s, cg, nbn, rtol, active);
 }
 

@@ -140,6 +140,7 @@ struct cmg_ctx {
     bool precond_built = false;   // the preconditioner holds a factorisation of this Newton solve
     int64_t sw_n_small = 0;
     int sw_nd_small = 0;
+    int64_t sw_inv_len = 0, sw_y_len = 0;   // inverse values and patch-vector rows of all patches
 };
 
 namespace cmg {
@@ -151,6 +152,16 @@ GatherMaps gather_maps(cmg_ctx* c);
 // leave the context with block Jacobi
 constexpr int kMaxPatchDofs = 72;
 int build_vertex_patches(cmg_ctx* c, const int32_t* elements);   // sets c->schwarz
-void launch_patch_setup(cmg_ctx* c, const void* vals);            // inverses of the patch matrices
-void launch_patch_apply(cmg_ctx* c, const double* r, double* z);  // z = sum_p R_p^T A_p^-1 R_p r
+// S parameter sets over the context's patches (blockIdx.y = set): per-set strides of the
+// block values, inverses and vectors; sets with flags[set] != 0 (setup: == 0) are skipped
+struct PatchSets {
+    int S = 1;
+    int64_t set_vals = 0, set_inv = 0, set_vec = 0, set_y = 0;
+    void* inv = nullptr;        // [S][sw_inv_len] (the context's sw_inv when null)
+    double* y = nullptr;        // [S][sw_y_len] (the context's sw_y when null)
+    const int32_t* flags = nullptr;
+};
+void launch_patch_setup(cmg_ctx* c, const void* vals, const PatchSets& ps = PatchSets());  // patch inverses
+// z = sum_p R_p^T A_p^-1 R_p r
+void launch_patch_apply(cmg_ctx* c, const double* r, double* z, const PatchSets& ps = PatchSets());
 }  // namespace cmg

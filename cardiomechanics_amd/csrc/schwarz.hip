@@ -35,9 +35,14 @@ constexpr int kWave = 64;
 template <typename VT, typename IT>
 __global__ __launch_bounds__(kWave) void k_patch_setup(int64_t n, const int32_t* order, const int32_t* pptr,
                                                        const int64_t* sptr, const int32_t* slots, const int64_t* iptr,
-                                                       const VT* vals, IT* inv, int nd_cap) {
+                                                       const VT* vals, IT* inv, int nd_cap, int64_t set_vals,
+                                                       int64_t set_inv, const int32_t* active) {
     extern __shared__ double lds[];
     if ((int64_t)blockIdx.x >= n) return;
+    const int64_t set = blockIdx.y;
+    if (active && !active[set]) return;
+    vals += set * set_vals;
+    inv += set * set_inv;
     const int64_t p = order[blockIdx.x];
     const int lane = threadIdx.x;
     const int np = pptr[p + 1] - pptr[p];
@@ -135,8 +140,14 @@ __global__ __launch_bounds__(kWave) void k_patch_setup(int64_t n, const int32_t*
 template <typename IT>
 __global__ __launch_bounds__(256) void k_patch_apply(int64_t P, const int32_t* pptr, const int32_t* pnodes,
                                                      const int64_t* iptr, const IT* inv, const double* r,
-                                                     double* y, int nd_max) {
+                                                     double* y, int nd_max, int64_t set_inv, int64_t set_vec,
+                                                     int64_t set_y, const int32_t* skip) {
     extern __shared__ double lds[];
+    const int64_t set = blockIdx.y;
+    if (skip && skip[set]) return;   // uniform per workgroup
+    inv += set * set_inv;
+    r += set * set_vec;
+    y += set * set_y;
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t p = (int64_t)blockIdx.x * 4 + w;
     double* rp = lds + (size_t)w * nd_max;
@@ -157,7 +168,12 @@ __global__ __launch_bounds__(256) void k_patch_apply(int64_t P, const int32_t* p
 
 // z[3n + c] = sum over the patches holding node n of their y rows, in patch order
 __global__ __launch_bounds__(256) void k_patch_gather(int64_t N, const int32_t* cptr, const int32_t* cpos,
-                                                      const double* y, double* z) {
+                                                      const double* y, double* z, int64_t set_y, int64_t set_vec,
+                                                      const int32_t* skip) {
+    const int64_t set = blockIdx.y;
+    if (skip && skip[set]) return;
+    y += set * set_y;
+    z += set * set_vec;
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -254,22 +270,26 @@ int build_vertex_patches(cmg_ctx* c, const int32_t* elements) {
     c->sw_nd_max = 3 * np_max;
     c->sw_n_small = n_small;
     c->sw_nd_small = std::min(kSmallPatchDofs, 3 * np_max);
+    c->sw_inv_len = iptr[P];
+    c->sw_y_len = 3 * (int64_t)pnodes.size();
     c->schwarz = true;
     return CMG_SUCCESS;
 }
 
-void launch_patch_setup(cmg_ctx* c, const void* vals) {
+void launch_patch_setup(cmg_ctx* c, const void* vals, const PatchSets& ps) {
     const int64_t cls_n[2] = {c->sw_n_small, c->sw_P - c->sw_n_small};
     const int cls_nd[2] = {c->sw_nd_small, c->sw_nd_max};
+    void* inv = ps.inv ? ps.inv : c->sw_inv.p;
     for (int k = 0; k < 2; k++) {
         if (cls_n[k] == 0) continue;
         const int nd = cls_nd[k];
         const size_t lds = (size_t)(nd * nd + nd) * sizeof(double) + (size_t)nd * sizeof(int);
         const int32_t* order = c->sw_order.as<int32_t>() + (k ? c->sw_n_small : 0);
-#define CMG_PSETUP(VT, IT)                                                                                        \
-    hipLaunchKernelGGL((k_patch_setup<VT, IT>), dim3((unsigned)cls_n[k]), dim3(kWave), lds, c->stream, cls_n[k], \
-                       order, c->sw_pptr.as<int32_t>(), c->sw_sptr.as<int64_t>(), c->sw_slots.as<int32_t>(),      \
-                       c->sw_iptr.as<int64_t>(), (const VT*)vals, c->sw_inv.as<IT>(), nd)
+        const dim3 grid((unsigned)cls_n[k], ps.S);
+#define CMG_PSETUP(VT, IT)                                                                                           \
+    hipLaunchKernelGGL((k_patch_setup<VT, IT>), grid, dim3(kWave), lds, c->stream, cls_n[k], order,                 \
+                       c->sw_pptr.as<int32_t>(), c->sw_sptr.as<int64_t>(), c->sw_slots.as<int32_t>(),              \
+                       c->sw_iptr.as<int64_t>(), (const VT*)vals, (IT*)inv, nd, ps.set_vals, ps.set_inv, ps.flags)
         if (c->fp32)
             CMG_PSETUP(float, float);
         else
@@ -278,20 +298,22 @@ void launch_patch_setup(cmg_ctx* c, const void* vals) {
     }
 }
 
-void launch_patch_apply(cmg_ctx* c, const double* r, double* z) {
+void launch_patch_apply(cmg_ctx* c, const double* r, double* z, const PatchSets& ps) {
     const int ndm = c->sw_nd_max;
-    const dim3 grid((unsigned)((c->sw_P + 3) / 4));
+    const dim3 grid((unsigned)((c->sw_P + 3) / 4), ps.S);
     const size_t lds = (size_t)4 * ndm * sizeof(double);
+    const void* inv = ps.inv ? ps.inv : c->sw_inv.p;
+    double* y = ps.y ? ps.y : c->sw_y.as<double>();
     if (c->fp32)
         hipLaunchKernelGGL(k_patch_apply<float>, grid, dim3(256), lds, c->stream, c->sw_P, c->sw_pptr.as<int32_t>(),
-                           c->sw_pnodes.as<int32_t>(), c->sw_iptr.as<int64_t>(), c->sw_inv.as<float>(), r,
-                           c->sw_y.as<double>(), ndm);
+                           c->sw_pnodes.as<int32_t>(), c->sw_iptr.as<int64_t>(), (const float*)inv, r, y, ndm,
+                           ps.set_inv, ps.set_vec, ps.set_y, ps.flags);
     else
         hipLaunchKernelGGL(k_patch_apply<double>, grid, dim3(256), lds, c->stream, c->sw_P, c->sw_pptr.as<int32_t>(),
-                           c->sw_pnodes.as<int32_t>(), c->sw_iptr.as<int64_t>(), c->sw_inv.as<double>(), r,
-                           c->sw_y.as<double>(), ndm);
-    hipLaunchKernelGGL(k_patch_gather, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, c->N,
-                       c->sw_cptr.as<int32_t>(), c->sw_cpos.as<int32_t>(), c->sw_y.as<double>(), z);
+                           c->sw_pnodes.as<int32_t>(), c->sw_iptr.as<int64_t>(), (const double*)inv, r, y, ndm,
+                           ps.set_inv, ps.set_vec, ps.set_y, ps.flags);
+    hipLaunchKernelGGL(k_patch_gather, dim3((unsigned)((c->N + 255) / 256), ps.S), dim3(256), 0, c->stream, c->N,
+                       c->sw_cptr.as<int32_t>(), c->sw_cpos.as<int32_t>(), y, z, ps.set_y, ps.set_vec, ps.flags);
 }
 
 }  // namespace cmg

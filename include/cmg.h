@@ -281,9 +281,9 @@ int cmg_spmv_layout(cmg_ctx* ctx, const double* values, const double* x, double*
  * layouts 0, 1 at 4/8/16 lanes and layout 2 at 16/32/64 lanes; the chosen layout and
  * lanes (all zero before the first solve) */
 int cmg_spmv_tuning(cmg_ctx* ctx, double* ms, int32_t* layout, int32_t* lanes);
-/* BiCGStab or GMRES(m) (cmg_options.krylov_method), right preconditioned with 3x3
- * nodal block Jacobi; replaces the KSP/PC of CBSolverNewmarkBeta.cpp:166-196 (a
- * Krylov solve instead of LU). */
+/* BiCGStab or GMRES(m) (cmg_options.krylov_method), right preconditioned
+ * (cmg_options.preconditioner); replaces the KSP/PC of CBSolverNewmarkBeta.cpp:166-196
+ * (a Krylov solve instead of LU). */
 int cmg_linear_solve(cmg_ctx* ctx, const double* values, const double* b, double* x, double rtol,
                      cmg_solve_stats* stats);
 
@@ -342,9 +342,10 @@ int cmg_set_land17_state(cmg_ctx* ctx, const double* state, double commit_time);
  * No counterpart in the reference, which solves one parameter set per run: S
  * copies of a context's quasi-static problem with their own material tables, one
  * mesh.  Each set follows cmg_static_step (CBSolverEquilibrium.cpp:234-363);
- * every kernel launch covers all sets, and the linear solve is a block-Jacobi
- * preconditioned conjugate gradient whose scalars stay on the device (runs of
- * iterations are replayed as one HIP graph).  CG needs symmetric matrices: the
+ * every kernel launch covers all sets, and the linear solve is a conjugate gradient,
+ * preconditioned like the context's solves (vertex-patch Schwarz with one set of patch
+ * inverses per parameter set, or 3x3 block Jacobi), whose scalars stay on the device
+ * (runs of iterations are replayed as one HIP graph).  CG needs symmetric matrices: the
  * assembled follower-pressure tangent of a closed T3 cavity whose rim is clamped
  * (the LV endocardium) is symmetric and is included when the context's
  * pressure_tangent option is set; for open or T6 pressure surfaces create the
