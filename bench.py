@@ -59,7 +59,7 @@ def main():
     ap.add_argument("--precond-reuse", type=int, default=None,
                     help="1: the beat window runs build the preconditioner once per Newton solve (default: the "
                          "library's)")
-    ap.add_argument("--beat-train", type=int, default=2,
+    ap.add_argument("--beat-train", type=int, default=1,
                     help="training windows for the warm start (Tmax +-15 %%, activation +-10 ms, seeded)")
     ap.add_argument("--beat-cpu-steps", type=int, default=1,
                     help="Newmark steps of the same beat run by the CPU reference solver for the beat's CPU "

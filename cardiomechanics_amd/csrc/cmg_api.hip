@@ -672,7 +672,9 @@ int to_device_coeffs(cmg_ctx* c, const double* h, int k) {
 bool use_schwarz(const cmg_ctx* c) { return c->schwarz; }
 
 void precond_setup(cmg_ctx* c, const void* vals) {
-    if (c->opt.precond_reuse && c->precond_built) return;   // lagged: the first tangent's
+    // lagged (Schwarz only: a block-Jacobi setup is one pass over the diagonal blocks, and
+    // reusing it cost the 200k beat window 50 % more Krylov iterations)
+    if (c->opt.precond_reuse && c->precond_built && use_schwarz(c)) return;
     c->precond_built = true;
     if (use_schwarz(c))
         launch_patch_setup(c, vals);
@@ -1001,11 +1003,13 @@ int bicgstab_device(cmg_ctx* c, const void* vals, const double* b, double* x, do
         launch_bicg_init(k, c->N, b, rtol_dev, active, s);
     };
     restart();
-    // Schwarz: the patch kernels run on every iteration of a replayed chunk, also after
-    // the recurrence finished (the update kernels then skip, so the extra applies are
-    // discarded)
+    // Schwarz: the patch kernels skip, like the update kernels, once the recurrence has
+    // finished within a replayed chunk (the done flag k.done)
     const PrecondApply apply = use_schwarz(c) ? [](void* cc, const double* u, double* z) {
-        launch_patch_apply(static_cast<cmg_ctx*>(cc), u, z);
+        cmg_ctx* cx = static_cast<cmg_ctx*>(cc);
+        PatchSets ps;
+        ps.flags = cx->bicg_flags.as<int32_t>();
+        launch_patch_apply(cx, u, z, ps);
     } : PrecondApply(nullptr);
     auto iterate = [&]() {
         launch_bicg_iteration(k, c->bicg_lanes, c->bicg_chunked, c->N, rp, ci, vals, c->fp32, 0,

@@ -152,10 +152,11 @@ typedef struct cmg_options {
                                    nodes of its edges, patch matrices inverted on the device;
                                    block Jacobi for T4); 0 (default):
                                    2 where it applies */
-    int32_t precond_reuse;      /* 1 (default): the preconditioner is built from the first tangent
-                                   of each Newton solve and reused by its later linear solves (the
-                                   reference's SNESSetLagPreconditioner(-2), CBSolverNewmarkBeta.cpp:
-                                   173-174); 0: built from every tangent */
+    int32_t precond_reuse;      /* 1 (default): the Schwarz preconditioner is built from the first
+                                   tangent of each Newton solve and reused by its later linear solves
+                                   (the reference's SNESSetLagPreconditioner(-2),
+                                   CBSolverNewmarkBeta.cpp:173-174); 0: built from every tangent.
+                                   Block Jacobi is always built from every tangent */
     double krylov_eta_max;      /* upper bound of the Eisenstat-Walker forcing term of the Newton
                                    linear solves (default 0.5; 0 reads as the default) */
 } cmg_options;
