@@ -654,6 +654,13 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
         else if (km == "BiCGStab") o.krylov_method = 2;
         else throw std::runtime_error("Solver.GPU.KrylovMethod must be GMRES, BiCGStab or BiCGStabGraph, not " + km);
     }
+    {
+        const std::string pc = p.Get<std::string>("Solver.GPU.Preconditioner", "Schwarz");
+        if (pc == "Schwarz") o.preconditioner = 0;   // vertex-patch Schwarz where it applies (T10)
+        else if (pc == "BlockJacobi") o.preconditioner = 1;
+        else throw std::runtime_error("Solver.GPU.Preconditioner must be Schwarz or BlockJacobi, not " + pc);
+    }
+    o.precond_reuse = p.Get<bool>("Solver.GPU.PreconditionerReuse", true) ? 1 : 0;
     o.transpose_tangent = p.Get<bool>("Solver.GPU.TransposeTangent", false) ? 1 : 0;
     o.pressure_tangent = p.Get<bool>("Solver.GPU.PressureTangent", true) ? 1 : 0;
     o.device = p.Get<int>("Solver.GPU.Device", 0);
