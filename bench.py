@@ -59,6 +59,10 @@ def main():
     ap.add_argument("--precond-reuse", type=int, default=None,
                     help="1: the beat window runs build the preconditioner once per Newton solve (default: the "
                          "library's)")
+    ap.add_argument("--newton-stall", type=int, default=5,
+                    help="Newton iterations of < 10 %% residual reduction in a row that end a beat-window step "
+                         "as failed, for every run of the window (0: run on to Solver.MaxIterations as the "
+                         "reference does)")
     ap.add_argument("--beat-train", type=int, default=1,
                     help="training windows for the warm start (Tmax +-15 %%, activation +-10 ms, seeded)")
     ap.add_argument("--beat-cpu-steps", type=int, default=1,
@@ -327,6 +331,8 @@ def beat_window_runs(args, mesh, mats):
         kw = {} if args.krylov_method is None else {"krylov_method": args.krylov_method}
         if args.precond_reuse is not None:
             kw["precond_reuse"] = args.precond_reuse
+        if args.newton_stall is not None:
+            kw["newton_stall"] = args.newton_stall
         with MechanicsContext(msh, [mat], precision=prec, preconditioner=pc, newton_atol=atol, newton_rtol=rtol,
                               newton_stol=stol, newmark_beta=0.25, newmark_gamma=0.5, **kw) as ctx:
             net = WarmStartNet(ctx, seed=5) if (weights is not None or record) else None
@@ -408,7 +414,8 @@ def beat_window_runs(args, mesh, mats):
             + (f", learned guess kept {r['kept']}/{r['proposed']}" if weights is not None else ""))
     out = {"workload": f"LV beat window 0-{args.beat_ms:g} ms, Newmark beta 1/4 gamma 1/2, dt 1 ms with CBTiming "
                        f"halving, -16 kPa sin^2 cavity pressure, 60 kPa sin^2 active tension apex->base",
-           "elements": mesh.n_elements, "newton_tolerance": [atol, rtol, stol], "runs": {}}
+           "elements": mesh.n_elements, "newton_tolerance": [atol, rtol, stol],
+           "newton_stall": args.newton_stall, "runs": {}}
     ref = runs.get("fp64")
     umax = max(float(np.abs(u).max()) for u in ref["states"].values()) if ref else None
     for lab, r in runs.items():

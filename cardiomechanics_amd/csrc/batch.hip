@@ -315,7 +315,7 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
     const int S = b->S;
     const int64_t n = b->n;
     const cmg_options& o = c->opt;
-    std::vector<int32_t> run(S, 1), its(S, 0), kry(S, 0), reason(S, 0), zero_tried(S, 0), corrupt;
+    std::vector<int32_t> run(S, 1), its(S, 0), kry(S, 0), reason(S, 0), zero_tried(S, 0), stall(S, 0), corrupt;
     std::vector<double> lam(S, 1.0), rprev(S, 0.0), r0(S, 0.0), rn(S, 0.0), sn(S, 0.0), dn(S, 0.0), eta(S, 1e-1);
     std::vector<double> eta_k(S, 1e-1);
     const double t0 = now_s();
@@ -389,6 +389,10 @@ int bnewton(cmg_batch* b, double time, cmg_step_stats* st) {
             else if (its[i] > 0 && rn[i] < o.newton_rtol * r0[i]) reason[i] = 3;
             else if (its[i] > 0 && rn[i] <= rprev[i] && sn[i] * lam[i] < o.newton_stol * dn[i]) reason[i] = 4;
             else if (its[i] >= o.newton_maxit) reason[i] = -5;
+            if (reason[i] == 0 && its[i] > 0) {
+                stall[i] = (rn[i] > 0.9 * rprev[i]) ? stall[i] + 1 : 0;   // as the single-set Newton
+                if (o.newton_stall > 0 && stall[i] >= o.newton_stall) reason[i] = -7;
+            }
             if (reason[i] != 0) {
                 run[i] = 0;
                 continue;

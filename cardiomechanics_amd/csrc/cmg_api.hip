@@ -186,6 +186,7 @@ void cmg_default_options(cmg_options* o) {
     o->preconditioner = 0;   // vertex-patch Schwarz for T10, block Jacobi otherwise
     o->krylov_eta_max = 0.5;
     o->precond_reuse = 1;   // as the reference lags it; 6 % (FP64) / 3 % (mixed) faster on the 200k beat window
+    o->newton_stall = 0;    // the reference runs a stalled solve on to Solver.MaxIterations
 }
 
 const char* cmg_last_error(void) { return g_last_error.c_str(); }
@@ -1362,6 +1363,9 @@ constexpr double kMinDomainLambda = 1.0 / (1 << 20);
 // SNES divergence tolerance (PETSc's default divtol): ||r|| > 1e4 ||r_0|| ends the
 // step as diverged (SNES_DIVERGED_DTOL)
 constexpr double kNewtonDivTol = 1e4;
+// a Newton iteration that keeps more than this fraction of the residual norm counts
+// towards cmg_options.newton_stall
+constexpr double kStallRatio = 0.9;
 
 // Rayleigh damping matrix C = alpha M + beta_R K(x_n) for the configuration in nodes
 // (CBSolverNewmarkBeta::CalcDampingMatrix, :657-702, called at the start of each new
@@ -1445,6 +1449,7 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
     double eta = 1e-1;
     double lambda = 1.0;
     bool zero_tried = false;
+    int stall = 0;
     c->precond_built = false;   // a new nonlinear solve builds its preconditioner afresh
     int it = 0;
     for (;; it++) {
@@ -1527,6 +1532,12 @@ int newton(cmg_ctx* c, double time, double cm, double cd, cmg_step_stats* st) {
         }
         if (it >= c->opt.newton_maxit) {
             st->converged_reason = -5;   // SNES_DIVERGED_MAX_IT
+            break;
+        }
+        // stalled: less than 10 % reduction per iteration, newton_stall times in a row
+        if (it > 0) stall = (rnorm > kStallRatio * rprev) ? stall + 1 : 0;
+        if (c->opt.newton_stall > 0 && stall >= c->opt.newton_stall) {
+            st->converged_reason = -7;
             break;
         }
         const double eta_max = c->opt.krylov_eta_max > 0.0 ? c->opt.krylov_eta_max : 0.5;

@@ -661,6 +661,7 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
         else throw std::runtime_error("Solver.GPU.Preconditioner must be Schwarz or BlockJacobi, not " + pc);
     }
     o.precond_reuse = p.Get<bool>("Solver.GPU.PreconditionerReuse", true) ? 1 : 0;
+    o.newton_stall = p.Get<int>("Solver.GPU.NewtonStall", o.newton_stall);
     o.transpose_tangent = p.Get<bool>("Solver.GPU.TransposeTangent", false) ? 1 : 0;
     o.pressure_tangent = p.Get<bool>("Solver.GPU.PressureTangent", true) ? 1 : 0;
     o.device = p.Get<int>("Solver.GPU.Device", 0);

@@ -159,6 +159,10 @@ typedef struct cmg_options {
                                    Block Jacobi is always built from every tangent */
     double krylov_eta_max;      /* upper bound of the Eisenstat-Walker forcing term of the Newton
                                    linear solves (default 0.5; 0 reads as the default) */
+    int32_t newton_stall;       /* k > 0: a Newton solve whose residual norm falls by less than 10 %
+                                   in each of k consecutive iterations ends as diverged (reason -7)
+                                   instead of running on to Solver.MaxIterations; 0: off */
+    int32_t reserved_i2;
 } cmg_options;
 
 typedef struct cmg_solve_stats {
@@ -173,7 +177,8 @@ typedef struct cmg_step_stats {
     int32_t krylov_iterations;   /* KSPGetTotalIterations */
     int32_t converged_reason;    /* >0 converged (2 atol, 3 rtol, 4 stol), <0 diverged (-3 the Krylov
                                     solve gave no usable update, -4 non-finite residual, -5
-                                    Solver.MaxIterations, -6 an element stays inverted, -9
+                                    Solver.MaxIterations, -6 an element stays inverted, -7 the
+                                    residual stalled (cmg_options.newton_stall), -9
                                     ||r|| > 1e4 ||r_0||) */
     int32_t corrupt_element;     /* first element with det F <= 0, or -1 */
     double residual_norm;        /* final ||r||_2 */

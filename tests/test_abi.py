@@ -33,7 +33,7 @@ def test_struct_layouts_match_header_sizes():
     # cmg_material: 4 int32 + density + 12 params + tension_max + 12 fn doubles
     assert ctypes.sizeof(_lib.CmgMaterial) == 16 + 8 * (1 + 12 + 1 + 12)
     assert ctypes.sizeof(_lib.CmgMesh) == 3 * 8 + 4 * 4 + 8 * 8
-    assert ctypes.sizeof(_lib.CmgOptions) == 10 * 4 + 7 * 8 + 4 * 8
+    assert ctypes.sizeof(_lib.CmgOptions) == 12 * 4 + 7 * 8 + 4 * 8
     assert ctypes.sizeof(_lib.CmgStepStats) == 4 * 4 + 6 * 8 + 2 * 4 + 8
     assert ctypes.sizeof(_lib.CmgBatchInfo) == 8 * 4 + 12 * 8
 
