@@ -86,12 +86,13 @@ class CmgStepStats(ctypes.Structure):
 class CmgBatchInfo(ctypes.Structure):
     _fields_ = [("n_sets", ctypes.c_int32), ("element_variant", ctypes.c_int32), ("spmv_lanes", ctypes.c_int32),
                 ("cg_chunk", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("tuned", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2), ("element_ms", ctypes.c_double * 8),
+                ("preconditioner", ctypes.c_int32), ("reserved", ctypes.c_int32), ("element_ms", ctypes.c_double * 8),
                 ("spmv_ms", ctypes.c_double * 3), ("tune_seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {"n_sets": self.n_sets, "element_variant": self.element_variant, "spmv_lanes": self.spmv_lanes,
                 "cg_chunk": self.cg_chunk, "use_graph": self.use_graph, "tuned": self.tuned,
+                "preconditioner": "vertex-patch Schwarz" if self.preconditioner == 2 else "block Jacobi",
                 "element_ms": list(self.element_ms), "spmv_ms": list(self.spmv_ms)}
 
 

@@ -499,9 +499,15 @@ int cmg_batch_create(cmg_ctx* c, int32_t n_sets, const cmg_material* materials, 
     CMG_TRY(alloc(b->vals, (size_t)S * c->nnzb * 9 * kt));
     CMG_TRY(alloc(b->dinv, (size_t)S * c->N * 9 * sizeof(double)));
     if (c->schwarz) {
-        CMG_TRY(alloc(b->sw_inv, (size_t)S * c->sw_inv_len * kt));
-        CMG_TRY(alloc(b->sw_y, (size_t)S * c->sw_y_len * sizeof(double)));
-        b->schwarz = true;
+        // S sets of patch inverses (S x 548 MB in FP64 on the 200k LV): where they do not
+        // fit, the batch keeps block Jacobi
+        b->schwarz = b->sw_inv.alloc((size_t)S * c->sw_inv_len * kt) &&
+                     b->sw_y.alloc((size_t)S * c->sw_y_len * sizeof(double));
+        if (!b->schwarz) {
+            b->sw_inv.release();
+            b->sw_y.release();
+            (void)hipGetLastError();   // the failed allocation is not an error of the batch
+        }
     }
     CMG_TRY(alloc(b->sc, (size_t)S * 6 * sizeof(double)));
     CMG_TRY(alloc(b->flags, (size_t)S * 2 * sizeof(int32_t)));
@@ -543,6 +549,7 @@ int cmg_batch_create(cmg_ctx* c, int32_t n_sets, const cmg_material* materials, 
     launch_fill_int(2 * S, 1, cg.done, s);
     std::memset(&b->info, 0, sizeof(b->info));
     b->info.n_sets = S;
+    b->info.preconditioner = b->schwarz ? 2 : 1;
     b->info.element_variant = c->opt.element_variant;
     b->info.spmv_lanes = 8;
     b->info.cg_chunk = 16;

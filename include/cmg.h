@@ -366,7 +366,8 @@ typedef struct cmg_batch_info {
     int32_t cg_chunk;          /* CG iterations per graph replay */
     int32_t use_graph;
     int32_t tuned;             /* 1 once the autotuner has run */
-    int32_t reserved[2];
+    int32_t preconditioner;    /* of the batched CG: 1 block Jacobi, 2 vertex-patch Schwarz */
+    int32_t reserved;
     double element_ms[8];      /* measured launch time of element variants 1..8 (0: not timed) */
     double spmv_ms[3];         /* measured batched SpMV time at 4, 8, 16 lanes */
     double tune_seconds;

@@ -67,6 +67,7 @@ def test_batch_static_steps_match_independent_solves():
     hist, info, vols = run_batch(mesh, sets, -1000.0, 8, **TIGHT)
     print("autotuner:", info)
     assert info["tuned"] == 1 and info["element_variant"] in range(1, 9) and info["spmv_lanes"] in (4, 8, 16)
+    assert info["preconditioner"] == "vertex-patch Schwarz"   # T10: the context built its patches
     for i, m in enumerate(sets):
         with MechanicsContext(mesh, [m], krylov_method=1, **TIGHT) as one:
             for k in range(9):
