@@ -45,11 +45,10 @@ __device__ __forceinline__ float bf16_round(float x) {
 }
 
 // ---------------------------------------------------------------------------
-// forward: grid-stride over 64-row tiles; W2 stays in LDS for all tiles of a block.
-// Thread t owns rows 4*(t/16)..+4 and hidden columns 8*(t%16)..+8.
+// forward (FP32): grid-stride over 64-row tiles; W2 stays in LDS for all tiles of a
+// block.  Thread t owns rows 4*(t/16)..+4 and hidden columns 8*(t%16)..+8.
 // store_act: write the hidden activations A1, A2 [rows][H] (training).
 // ---------------------------------------------------------------------------
-template <bool BF16>
 __global__ __launch_bounds__(kThr) void k_mlp_forward(const float* __restrict__ w, const float* __restrict__ X,
                                                       int64_t rows, float* __restrict__ Y, float* __restrict__ A1g,
                                                       float* __restrict__ A2g) {
@@ -60,9 +59,9 @@ __global__ __launch_bounds__(kThr) void k_mlp_forward(const float* __restrict__ 
     __shared__ float sb3[kOut];
     __shared__ float sA[kTile * kPad];
     const int t = threadIdx.x;
-    for (int i = t; i < kH * kH; i += kThr) sW2[i] = BF16 ? bf16_round(w[off_W2 + i]) : w[off_W2 + i];
-    for (int i = t; i < kIn * kH; i += kThr) sW1[i] = BF16 ? bf16_round(w[i]) : w[i];
-    for (int i = t; i < kH * kOut; i += kThr) sW3[i] = BF16 ? bf16_round(w[off_W3 + i]) : w[off_W3 + i];
+    for (int i = t; i < kH * kH; i += kThr) sW2[i] = w[off_W2 + i];
+    for (int i = t; i < kIn * kH; i += kThr) sW1[i] = w[i];
+    for (int i = t; i < kH * kOut; i += kThr) sW3[i] = w[off_W3 + i];
     if (t < kH) {
         sb1[t] = w[off_b1 + t];
         sb2[t] = w[off_b2 + t];
@@ -80,10 +79,7 @@ __global__ __launch_bounds__(kThr) void k_mlp_forward(const float* __restrict__ 
             const int64_t r = r0 + rg * 4 + i;
             float x[kIn];
 #pragma unroll
-            for (int k = 0; k < kIn; k++) {
-                const float v = (r < rows) ? X[r * kIn + k] : 0.f;
-                x[k] = BF16 ? bf16_round(v) : v;
-            }
+            for (int k = 0; k < kIn; k++) x[k] = (r < rows) ? X[r * kIn + k] : 0.f;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int c = cg * 8 + j;
@@ -98,7 +94,7 @@ __global__ __launch_bounds__(kThr) void k_mlp_forward(const float* __restrict__ 
             const int64_t r = r0 + rg * 4 + i;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                sA[(rg * 4 + i) * kPad + cg * 8 + j] = BF16 ? bf16_round(acc[i][j]) : acc[i][j];
+                sA[(rg * 4 + i) * kPad + cg * 8 + j] = acc[i][j];
                 if (A1g && r < rows) A1g[r * kH + cg * 8 + j] = acc[i][j];
             }
         }
@@ -126,7 +122,7 @@ __global__ __launch_bounds__(kThr) void k_mlp_forward(const float* __restrict__ 
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const float h = tanhf(acc[i][j]);
-                sA[(rg * 4 + i) * kPad + cg * 8 + j] = BF16 ? bf16_round(h) : h;
+                sA[(rg * 4 + i) * kPad + cg * 8 + j] = h;
                 if (A2g && r < rows) A2g[r * kH + cg * 8 + j] = h;
             }
         }
@@ -140,6 +136,106 @@ __global__ __launch_bounds__(kThr) void k_mlp_forward(const float* __restrict__ 
             if (r < rows) Y[r * kOut + o] = s;
         }
         __syncthreads();   // sA is rewritten by the next tile
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bfloat16 inference: the 128 x 128 layer on the matrix cores
+// (v_mfma_f32_16x16x16_bf16, FP32 accumulation), layers 1 and 3 in FP32 on the vector
+// ALUs; inputs, weights and both hidden activations rounded to bfloat16.  One wave per
+// 16 rows (four waves, 64 rows per tile).  Operand layout of the instruction (index
+// m % 4 is the item within a lane's four values):
+//   A[i][k] in lane i + 16 (k / 4),  B[k][j] in lane j + 16 (k / 4),  D[i][j] in lane j + 16 (i / 4).
+// A lane therefore computes its own layer-1 values straight into its A operands
+// (row i = lane % 16, columns 16 ks + 4 (lane / 16) + 0..3 for k-step ks), reads B from
+// W2 stored transposed in LDS (four consecutive k in 8 bytes), and ends with the
+// layer-2 outputs of rows 4 (lane / 16) + 0..3 and columns 16 nb + lane % 16, whose
+// layer-3 products are summed over the 16 lanes of a row group.
+// ---------------------------------------------------------------------------
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kW2tPitch = kH + 8;   // shorts per row of the transposed W2 (spreads the lanes over banks)
+
+__device__ __forceinline__ short bf16_bits(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
+    return (short)(u >> 16);
+}
+
+__global__ __launch_bounds__(kThr) void k_mlp_forward_mfma(const float* __restrict__ w, const float* __restrict__ X,
+                                                           int64_t rows, float* __restrict__ Y) {
+    __shared__ __align__(16) short sW2t[kH * kW2tPitch];   // sW2t[n * pitch + k] = bf16(W2[k][n])
+    __shared__ float sW1[kIn * kH];
+    __shared__ float sb1[kH], sb2[kH];
+    __shared__ float sW3[kH * kOut];
+    __shared__ float sb3[kOut];
+    const int t = threadIdx.x;
+    for (int i = t; i < kH * kH; i += kThr) sW2t[(i % kH) * kW2tPitch + i / kH] = bf16_bits(w[off_W2 + i]);
+    for (int i = t; i < kIn * kH; i += kThr) sW1[i] = bf16_round(w[i]);
+    for (int i = t; i < kH * kOut; i += kThr) sW3[i] = bf16_round(w[off_W3 + i]);
+    if (t < kH) {
+        sb1[t] = w[off_b1 + t];
+        sb2[t] = w[off_b2 + t];
+    }
+    if (t < kOut) sb3[t] = w[off_b3 + t];
+    __syncthreads();
+    const int wv = t / 64, lane = t % 64;
+    const int li = lane % 16, lk = lane / 16;
+    const int64_t ntiles = (rows + kTile - 1) / kTile;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t r0 = tile * kTile + wv * 16;
+        // ---- layer 1 into the A operands ----
+        const int64_t ra = r0 + li;
+        float x[kIn];
+#pragma unroll
+        for (int k = 0; k < kIn; k++) x[k] = (ra < rows) ? bf16_round(X[ra * kIn + k]) : 0.f;
+        bf16x4 a[kH / 16];
+#pragma unroll
+        for (int ks = 0; ks < kH / 16; ks++)
+#pragma unroll 1
+            for (int j = 0; j < 4; j++) {
+                const int c = 16 * ks + 4 * lk + j;
+                float s = sb1[c];
+#pragma unroll
+                for (int k = 0; k < kIn; k++) s += x[k] * sW1[k * kH + c];
+                a[ks][j] = bf16_bits(tanhf(s));
+            }
+        // ---- layers 2 (matrix cores) and 3, one 16-column block of layer 2 at a time ----
+        float y[4][kOut];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int o = 0; o < kOut; o++) y[r][o] = 0.f;
+#pragma unroll 1
+        for (int nb = 0; nb < kH / 16; nb++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < kH / 16; ks++) {
+                const bf16x4 b = *reinterpret_cast<const bf16x4*>(&sW2t[(16 * nb + li) * kW2tPitch + 16 * ks + 4 * lk]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ks], b, acc, 0, 0, 0);
+            }
+            const int n = 16 * nb + li;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float h = bf16_round(tanhf(acc[r] + sb2[n]));
+#pragma unroll
+                for (int o = 0; o < kOut; o++) y[r][o] += h * sW3[n * kOut + o];
+            }
+        }
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int o = 0; o < kOut; o++) y[r][o] += __shfl_xor(y[r][o], off, 16);
+        if (li == 0)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = r0 + 4 * lk + r;
+                if (row < rows)
+#pragma unroll
+                    for (int o = 0; o < kOut; o++) Y[row * kOut + o] = y[r][o] + sb3[o];
+            }
     }
 }
 
@@ -496,11 +592,10 @@ int cmg_mlp_forward(cmg_mlp* m, const float* X, int64_t rows, int32_t bf16, floa
     CMG_HIP_CHECK(hipSetDevice(m->c->device));
     hipStream_t s = m->c->stream;
     if (bf16)
-        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(fwd_blocks(rows)), dim3(kThr), 0, s, m->w.as<float>(), X, rows, Y,
-                           nullptr, nullptr);
+        hipLaunchKernelGGL(k_mlp_forward_mfma, dim3(fwd_blocks(rows)), dim3(kThr), 0, s, m->w.as<float>(), X, rows, Y);
     else
-        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(fwd_blocks(rows)), dim3(kThr), 0, s, m->w.as<float>(), X, rows,
-                           Y, nullptr, nullptr);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_blocks(rows)), dim3(kThr), 0, s, m->w.as<float>(), X, rows, Y,
+                           nullptr, nullptr);
     CMG_HIP_CHECK(hipGetLastError());
     return CMG_SUCCESS;
 }
@@ -515,7 +610,7 @@ int cmg_mlp_loss_grad(cmg_mlp* m, const float* X, const float* T, int64_t rows, 
     const float* w = m->w.as<float>();
     float* g = m->g.as<float>();
     const unsigned nb = fwd_blocks(rows);
-    hipLaunchKernelGGL(k_mlp_forward<false>, dim3(nb), dim3(kThr), 0, s, w, X, rows, m->Yt.as<float>(),
+    hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(kThr), 0, s, w, X, rows, m->Yt.as<float>(),
                        m->A1.as<float>(), m->A2.as<float>());
     hipLaunchKernelGGL(k_mlp_backward_act, dim3(nb), dim3(kThr), 0, s, w, rows, m->Yt.as<float>(), T,
                        m->A1.as<float>(), m->A2.as<float>(), m->dY.as<float>(), m->dZ2.as<float>(),

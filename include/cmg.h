@@ -420,7 +420,8 @@ int cmg_mlp_set_scales(cmg_mlp* mlp, double inc_scale, double pressure_rate_scal
 int cmg_mlp_features(cmg_mlp* mlp, double time, double pressure_rate, float* X);
 /* scaled increment of the last accepted Newmark step -> T [N][3] */
 int cmg_mlp_targets(cmg_mlp* mlp, float* T);
-/* Y = net(X); bf16 = 1 rounds weights and activations to bfloat16 (FP32 accumulation) */
+/* Y = net(X); bf16 = 1: inputs, weights and hidden activations in bfloat16, the 128 x 128
+ * layer on the matrix cores (FP32 accumulation) */
 int cmg_mlp_forward(cmg_mlp* mlp, const float* X, int64_t rows, int32_t bf16, float* Y);
 /* mean squared error of net(X) against T and its gradient (kept for cmg_mlp_adam_step;
  * copied to grad [num_weights] on the host when grad is not NULL) */
