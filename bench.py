@@ -350,7 +350,7 @@ def beat_window_runs(args, mesh, mats):
                 if record and ok_dt and full >= 2:
                     net.record_before_step(t + dt, rate)
                 if weights is not None and ok_dt and full >= 2:
-                    net.propose(t + dt, rate)
+                    net.propose(t + dt, rate, bf16=True)   # bfloat16 inference on the matrix cores
                     res["proposed"] += 1
                 rc, st = ctx.newmark_step(t + dt, dt)
                 res["newton"] += st.newton_iterations
@@ -423,7 +423,8 @@ def beat_window_runs(args, mesh, mats):
         rec["precision"] = "mixed" if conf[lab][0] else "fp64"
         rec["preconditioner"] = "block Jacobi" if conf[lab][1] == 1 else "vertex-patch Schwarz"
         if r["train"]:
-            rec["warm_start"] = dict(r["train"], kept=r["kept"], proposed=r["proposed"])
+            rec["warm_start"] = dict(r["train"], kept=r["kept"], proposed=r["proposed"],
+                                     inference="bf16 (v_mfma_f32_16x16x16_bf16), FP32 training")
         if ref is not None and lab != "fp64":
             common = sorted(set(ref["states"]) & set(r["states"]))
             if common:
