@@ -811,128 +811,128 @@ __global__ __launch_bounds__(64 * BSPLIT * GROUPS) void k_residual_tangent_v2(co
     const int b0 = bh * BH;
     TT keep[STAGED ? AH * BH : 1];   // V K_e entries of this lane for the staged store
     if (valid) {
-    const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
-    const bool guc = m.law == CMG_LAW_GUCCIONE;
-    TT bmat[9];
-    {
-        const TT bf = (TT)m.p[1], bt = (TT)m.p[2], bfs = (TT)m.p[3];
-        const TT bb[9] = {bf, bfs, bfs, bfs, bt, bt, bfs, bt, bt};
+        const DevMaterial& m = a.mats[a.matidx ? a.matidx[e] : 0];
+        const bool guc = m.law == CMG_LAW_GUCCIONE;
+        TT bmat[9];
+        {
+            const TT bf = (TT)m.p[1], bt = (TT)m.p[2], bfs = (TT)m.p[3];
+            const TT bb[9] = {bf, bfs, bfs, bfs, bt, bt, bfs, bt, bt};
 #pragma unroll
-        for (int r = 0; r < 9; r++) bmat[r] = bb[r];
-    }
-    const double V = a.w[e];
-    TT* Ke = static_cast<TT*>(a.Ke) + e * (9 * NPE * NPE);
+            for (int r = 0; r < 9; r++) bmat[r] = bb[r];
+        }
+        const double V = a.w[e];
+        TT* Ke = static_cast<TT*>(a.Ke) + e * (9 * NPE * NPE);
 #pragma unroll 1
-    for (int half = 0; half < NSPLIT; half++) {
-        TT acc[AH * BH];
+        for (int half = 0; half < NSPLIT; half++) {
+            TT acc[AH * BH];
 #pragma unroll
-        for (int r = 0; r < AH * BH; r++) acc[r] = (TT)0;
+            for (int r = 0; r < AH * BH; r++) acc[r] = (TT)0;
 #pragma unroll 1
-        for (int q = 0; q < NQP; q++) {
-            const TT* rec = sQ[slot][q];
-            TT u[3], w[3], Z[9], S[9], YX[9];
+            for (int q = 0; q < NQP; q++) {
+                const TT* rec = sQ[slot][q];
+                TT u[3], w[3], Z[9], S[9], YX[9];
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                u[r] = rec[3 * ii + r];
-                w[r] = rec[3 * jj + r];
-            }
+                for (int r = 0; r < 3; r++) {
+                    u[r] = rec[3 * ii + r];
+                    w[r] = rec[3 * jj + r];
+                }
 #pragma unroll
-            for (int r = 0; r < 9; r++) {
-                S[r] = rec[18 + r];
-                Z[r] = rec[27 + r];
-                YX[r] = rec[36 + r];
-            }
-            const TT bbt = rec[9 + ij], czz = rec[45], codot = rec[46], ceq = rec[47];
-            const TT d00 = rec[48], d11 = rec[49], mfs = rec[50];
-            TT Zu[3], Zw[3], Xu[3], Yw[3];
+                for (int r = 0; r < 9; r++) {
+                    S[r] = rec[18 + r];
+                    Z[r] = rec[27 + r];
+                    YX[r] = rec[36 + r];
+                }
+                const TT bbt = rec[9 + ij], czz = rec[45], codot = rec[46], ceq = rec[47];
+                const TT d00 = rec[48], d11 = rec[49], mfs = rec[50];
+                TT Zu[3], Zw[3], Xu[3], Yw[3];
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                Zu[r] = Z[3 * r] * u[0] + Z[3 * r + 1] * u[1] + Z[3 * r + 2] * u[2];
-                Zw[r] = Z[3 * r] * w[0] + Z[3 * r + 1] * w[1] + Z[3 * r + 2] * w[2];
-                Xu[r] = YX[3 * r] * u[0] + YX[3 * r + 1] * u[1] + YX[3 * r + 2] * u[2];
-                Yw[r] = YX[3 * r] * w[0] + YX[3 * r + 1] * w[1] + YX[3 * r + 2] * w[2];
-            }
-            const TT uZw = u[0] * Zw[0] + u[1] * Zw[1] + u[2] * Zw[2];
-            TT M[9];
-#pragma unroll
-            for (int mm = 0; mm < 3; mm++)
-#pragma unroll
-                for (int P = 0; P < 3; P++)
-                    M[3 * mm + P] = bbt * S[3 * P + mm] + czz * Zu[mm] * Zw[P] +
-                                    codot * (uZw * Z[3 * mm + P] + Zu[P] * Zw[mm]);
-            if (guc) {
-                const TT hc = (TT)0.5 * ceq, oc = (TT)2.0 * ceq;
-                TT h[3];
-#pragma unroll
-                for (int r = 0; r < 3; r++)
-                    h[r] = u[0] * w[0] * bmat[r] + u[1] * w[1] * bmat[3 + r] + u[2] * w[2] * bmat[6 + r];
+                for (int r = 0; r < 3; r++) {
+                    Zu[r] = Z[3 * r] * u[0] + Z[3 * r + 1] * u[1] + Z[3 * r + 2] * u[2];
+                    Zw[r] = Z[3 * r] * w[0] + Z[3 * r + 1] * w[1] + Z[3 * r + 2] * w[2];
+                    Xu[r] = YX[3 * r] * u[0] + YX[3 * r + 1] * u[1] + YX[3 * r + 2] * u[2];
+                    Yw[r] = YX[3 * r] * w[0] + YX[3 * r + 1] * w[1] + YX[3 * r + 2] * w[2];
+                }
+                const TT uZw = u[0] * Zw[0] + u[1] * Zw[1] + u[2] * Zw[2];
+                TT M[9];
 #pragma unroll
                 for (int mm = 0; mm < 3; mm++)
 #pragma unroll
                     for (int P = 0; P < 3; P++)
-                        M[3 * mm + P] += hc * (u[P] * w[mm] * bmat[3 * P + mm] + ((mm == P) ? h[mm] : (TT)0)) +
-                                         oc * Xu[mm] * Yw[P];
-            } else {
-                const TT av[3] = {u[1], u[0], (TT)0}, bv[3] = {w[1], w[0], (TT)0};
+                        M[3 * mm + P] = bbt * S[3 * P + mm] + czz * Zu[mm] * Zw[P] +
+                                        codot * (uZw * Z[3 * mm + P] + Zu[P] * Zw[mm]);
+                if (guc) {
+                    const TT hc = (TT)0.5 * ceq, oc = (TT)2.0 * ceq;
+                    TT h[3];
 #pragma unroll
-                for (int mm = 0; mm < 3; mm++)
+                    for (int r = 0; r < 3; r++)
+                        h[r] = u[0] * w[0] * bmat[r] + u[1] * w[1] * bmat[3 + r] + u[2] * w[2] * bmat[6 + r];
 #pragma unroll
-                    for (int P = 0; P < 3; P++) M[3 * mm + P] += Xu[mm] * w[P] + mfs * av[mm] * bv[P];
-                M[4] += d11 * u[1] * w[1];
-            }
-            M[0] += d00 * u[0] * w[0];
-            TT G[NPE][3];
+                    for (int mm = 0; mm < 3; mm++)
 #pragma unroll
-            for (int k = 0; k < NPE; k++)
+                        for (int P = 0; P < 3; P++)
+                            M[3 * mm + P] += hc * (u[P] * w[mm] * bmat[3 * P + mm] + ((mm == P) ? h[mm] : (TT)0)) +
+                                             oc * Xu[mm] * Yw[P];
+                } else {
+                    const TT av[3] = {u[1], u[0], (TT)0}, bv[3] = {w[1], w[0], (TT)0};
 #pragma unroll
-                for (int r = 0; r < 3; r++) G[k][r] = sG[slot][q][3 * k + r];
-            // b range of this lane with a compile-time offset (no dynamic register indexing)
-            auto contract = [&](auto B0) {
-                constexpr int kB0 = decltype(B0)::value;
+                    for (int mm = 0; mm < 3; mm++)
 #pragma unroll
-                for (int bb = 0; bb < BH; bb++) {
-                    const int b = kB0 + bb;
-                    const TT mg0 = M[0] * G[b][0] + M[1] * G[b][1] + M[2] * G[b][2];
-                    const TT mg1 = M[3] * G[b][0] + M[4] * G[b][1] + M[5] * G[b][2];
-                    const TT mg2 = M[6] * G[b][0] + M[7] * G[b][1] + M[8] * G[b][2];
-#pragma unroll
-                    for (int aa = 0; aa < AH; aa++) {
-                        const int ga = half * AH + aa;
-                        // three chained fused multiply-adds into the accumulator
-                        TT t = acc[aa * BH + bb];
-                        t = fma(G[ga][0], mg0, t);
-                        t = fma(G[ga][1], mg1, t);
-                        acc[aa * BH + bb] = fma(G[ga][2], mg2, t);
-                    }
+                        for (int P = 0; P < 3; P++) M[3 * mm + P] += Xu[mm] * w[P] + mfs * av[mm] * bv[P];
+                    M[4] += d11 * u[1] * w[1];
                 }
-            };
-            if (BSPLIT == 1 || bh == 0)
-                contract(std::integral_constant<int, 0>());
-            else
-                contract(std::integral_constant<int, BH>());
+                M[0] += d00 * u[0] * w[0];
+                TT G[NPE][3];
+#pragma unroll
+                for (int k = 0; k < NPE; k++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) G[k][r] = sG[slot][q][3 * k + r];
+                // b range of this lane with a compile-time offset (no dynamic register indexing)
+                auto contract = [&](auto B0) {
+                    constexpr int kB0 = decltype(B0)::value;
+#pragma unroll
+                    for (int bb = 0; bb < BH; bb++) {
+                        const int b = kB0 + bb;
+                        const TT mg0 = M[0] * G[b][0] + M[1] * G[b][1] + M[2] * G[b][2];
+                        const TT mg1 = M[3] * G[b][0] + M[4] * G[b][1] + M[5] * G[b][2];
+                        const TT mg2 = M[6] * G[b][0] + M[7] * G[b][1] + M[8] * G[b][2];
+#pragma unroll
+                        for (int aa = 0; aa < AH; aa++) {
+                            const int ga = half * AH + aa;
+                            // three chained fused multiply-adds into the accumulator
+                            TT t = acc[aa * BH + bb];
+                            t = fma(G[ga][0], mg0, t);
+                            t = fma(G[ga][1], mg1, t);
+                            acc[aa * BH + bb] = fma(G[ga][2], mg2, t);
+                        }
+                    }
+                };
+                if (BSPLIT == 1 || bh == 0)
+                    contract(std::integral_constant<int, 0>());
+                else
+                    contract(std::integral_constant<int, BH>());
+            }
+            const TT Vt = (TT)V;
+            if constexpr (STAGED) {
+#pragma unroll
+                for (int r = 0; r < AH * BH; r++) keep[r] = Vt * acc[r];
+            } else {
+#pragma unroll
+                for (int aa = 0; aa < AH; aa++)
+#pragma unroll
+                    for (int bb = 0; bb < BH; bb++)
+                        Ke[9 * ((half * AH + aa) * NPE + b0 + bb) + ij] = Vt * acc[aa * BH + bb];
+            }
         }
-        const TT Vt = (TT)V;
-        if constexpr (STAGED) {
+        if (jj == 0 && bh == 0) {
+            double* fe = a.fe + e * (3 * NPE);
 #pragma unroll
-            for (int r = 0; r < AH * BH; r++) keep[r] = Vt * acc[r];
-        } else {
+            for (int k = 0; k < NPE; k++) {
+                double t = 0.0;
 #pragma unroll
-            for (int aa = 0; aa < AH; aa++)
-#pragma unroll
-                for (int bb = 0; bb < BH; bb++)
-                    Ke[9 * ((half * AH + aa) * NPE + b0 + bb) + ij] = Vt * acc[aa * BH + bb];
+                for (int q = 0; q < NQP; q++) t += sT[slot][q][3 * k + ii];
+                fe[3 * k + ii] = V * t;
+            }
         }
-    }
-    if (jj == 0 && bh == 0) {
-        double* fe = a.fe + e * (3 * NPE);
-#pragma unroll
-        for (int k = 0; k < NPE; k++) {
-            double t = 0.0;
-#pragma unroll
-            for (int q = 0; q < NQP; q++) t += sT[slot][q][3 * k + ii];
-            fe[3 * k + ii] = V * t;
-        }
-    }
     }   // valid
     if constexpr (STAGED) {
         // 16-byte units: RA rows of one element are RA * ROW values, a whole number of units,
