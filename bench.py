@@ -60,9 +60,9 @@ def main():
                     help="1: the beat window runs build the preconditioner once per Newton solve (default: the "
                          "library's)")
     ap.add_argument("--newton-stall", type=int, default=5,
-                    help="Newton iterations of < 10 %% residual reduction in a row that end a beat-window step "
-                         "as failed, for every run of the window (0: run on to Solver.MaxIterations as the "
-                         "reference does)")
+                    help="Newton iterations of < 10 %% residual reduction in a row that end a step as failed, "
+                         "for every run of the beat window and of --beat-full (0: run on to Solver.MaxIterations "
+                         "as the reference does)")
     ap.add_argument("--beat-train", type=int, default=1,
                     help="training windows for the warm start (Tmax +-15 %%, activation +-10 ms, seeded)")
     ap.add_argument("--beat-cpu-steps", type=int, default=1,
@@ -493,7 +493,7 @@ def beat_full(args):
         if label not in labels:
             continue
         ctx = MechanicsContext(mesh, mats, precision=prec, rayleigh_alpha=alpha, rayleigh_beta=beta_r,
-                               newmark_beta=nm_beta, newmark_gamma=nm_gamma)
+                               newmark_beta=nm_beta, newmark_gamma=nm_gamma, newton_stall=args.newton_stall)
         states, stats, umax = {}, [], [0.0]
 
         def on(rec, st, ctx=ctx, states=states, stats=stats, label=label, umax=umax):
@@ -525,7 +525,7 @@ def beat_full(args):
            "config": {"workload": "LV beat, Holzapfel-Ogden + active tension, Newmark dt 1 ms with halving",
                       "elements": mesh.n_elements, "dofs": 3 * mesh.n_nodes, "end_time_s": args.beat_end,
                       "rayleigh_alpha": alpha, "rayleigh_beta": beta_r, "newmark_beta": nm_beta,
-                      "newmark_gamma": nm_gamma},
+                      "newmark_gamma": nm_gamma, "newton_stall": args.newton_stall},
            "value": runs[last][2],
            "dtype": "mixed (fp32 tangent and Krylov operator, fp64 residual)" if last == "mixed" else "f64",
            "wall_s": {k: v[2] for k, v in runs.items()},

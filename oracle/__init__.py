@@ -21,6 +21,16 @@ Parity status of each oracle function (what pins it; see tests/test_oracle_*.py)
   T3/T6 forces, volume      cube face force = p A n; closed-surface volume independent
                             of the reference point; tetra volume closed form
   Newton / Newmark drivers  linear spring-mass closed form; residual at convergence
+  SinusDriver, DoubleHill   values at chosen times against their closed forms
+  TanH tension              limits, window, periodicity and saturation; complex-step
+                            element tangent against the reference-style FD tangent
+  Land17 tension            closed-form steady state of the ODE system at constant
+                            calcium and stretch; closed-form first step from the initial
+                            state; calcium lookup rules; dT/dlambda by central differences
+  Rayleigh damping          C w = alpha M w + beta_R (reference-layout FD Jacobian) w;
+                            the damped oscillator in the modal limit
+  CPU reference solver      the FD epsilon (1 + |d|_2) eps and the Newmark / static steps
+                            (cpu_solver.cpp) against the drivers' roots
 """
 from __future__ import annotations
 
