@@ -296,7 +296,7 @@ def main():
 
 
 def beat_window_runs(args, mesh, mats):
-    """Config 2's beat on the LV mesh over [0, --beat-ms]: Newmark beta 1/4, gamma 1/2, dt 1 ms
+    """The 200k-element LV beat over [0, --beat-ms]: Newmark beta 1/4, gamma 1/2, dt 1 ms
     halved by CBTiming on failed steps (CBTiming.cpp:77-136), cavity pressure -16 kPa
     sin^2 over 0.8 s, active tension 60 kPa sin^2 from the apex-to-base activation, all
     runs at the same Newton tolerance.  Runs: FP64 with block Jacobi (the first GPU path),

@@ -98,7 +98,7 @@ BETA, GAMMA = 0.25, 0.5
 
 
 def newmark_window(mesh, prec, tol, t_end, warm=None, on_record=None):
-    """config 2's beat from t = 0 (Newmark 1/4, 1/2, dt 1 ms halved by CBTiming): per
+    """the LV beat from t = 0 (Newmark 1/4, 1/2, dt 1 ms halved by CBTiming): per
     accepted step the state before and after, the step's time, dt, pressure and stats"""
     from cardiomechanics_amd import solver as SV
 
@@ -169,8 +169,8 @@ def lv200k_windows(lv200k):
 
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
 def test_lv200k_beat_window_steps_are_oracle_roots(lv200k, lv200k_windows, prec):
-    """Config 2 (FP64) and config 3 (mixed precision) at full size: every accepted step of
-    the 200k-element LV beat through activation onset is a root of the oracle's Newmark
+    """The 200k-element LV beat in FP64 and in mixed precision: every accepted step of
+    the window through activation onset is a root of the oracle's Newmark
     equations at the GPU's state, to the step's tolerance."""
     _, cpu = lv200k
     tol, runs = lv200k_windows
@@ -178,7 +178,7 @@ def test_lv200k_beat_window_steps_are_oracle_roots(lv200k, lv200k_windows, prec)
 
 
 def test_lv200k_mixed_precision_displacements_match_fp64(lv200k, lv200k_windows):
-    """Config 3: at every time both runs accepted, the mixed-precision nodes differ from
+    """The 200k-element LV beat: at every time both runs accepted, the mixed-precision nodes differ from
     the FP64 nodes by <= 1e-8 of the largest displacement in the window."""
     mesh, _ = lv200k
     tol, runs = lv200k_windows
@@ -196,7 +196,7 @@ def test_lv200k_mixed_precision_displacements_match_fp64(lv200k, lv200k_windows)
 
 
 def test_lv1m_element_kernel_and_warm_started_newmark_steps():
-    """Config 5's mesh (the LV refined to ~1M T10 elements): sampled element residuals
+    """The LV refined to ~1M T10 elements: sampled element residuals
     against the oracle, then the beat's first Newmark steps where, from the fourth on,
     the learned warm start (trained on the run's own earlier steps) proposes the Newton
     guess; every accepted step is a root of the oracle's Newmark equations."""
@@ -265,4 +265,4 @@ def test_lv1m_element_kernel_and_warm_started_newmark_steps():
     check_roots(cpu, recs, tol, "1M LV")
     guesses = [r["guess"] for r in recs]
     print(f"Newton guess sources per step: {guesses}")
-    assert any(g in (2, 3) for g in guesses)      # the learned guess was proposed and judged
+    assert any(g in (2, 3) for g in guesses)      # the learned guess was proposed and checked
