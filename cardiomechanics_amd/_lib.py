@@ -62,7 +62,7 @@ class CmgOptions(ctypes.Structure):
                 ("fd_epsilon", ctypes.c_double), ("rayleigh_alpha", ctypes.c_double),
                 ("rayleigh_beta", ctypes.c_double), ("preconditioner", ctypes.c_int32),
                 ("precond_reuse", ctypes.c_int32), ("krylov_eta_max", ctypes.c_double),
-                ("newton_stall", ctypes.c_int32), ("reserved_i2", ctypes.c_int32)]
+                ("newton_stall", ctypes.c_int32), ("spmv_choice", ctypes.c_int32)]
 
 
 class CmgSolveStats(ctypes.Structure):

@@ -224,6 +224,7 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
         c->opt = *options;
     else
         cmg_default_options(&c->opt);
+    if (c->opt.spmv_choice < 0 || c->opt.spmv_choice > 9) return fail("cmg_options.spmv_choice must be 0-9");
     if (c->opt.element_variant < 1 || c->opt.element_variant > 8)
         c->opt.element_variant = (c->opt.precision == CMG_PREC_MIXED) ? 5 : 6;
     if (!std::isfinite(c->opt.rayleigh_alpha) || !std::isfinite(c->opt.rayleigh_beta))
@@ -922,6 +923,16 @@ int tune_spmv(cmg_ctx* c, const void* vals, const double* b) {
     float best = 1e30f, best_blk = 1e30f;
     const int64_t* rp = c->rowptr.as<int64_t>();
     const int32_t* ci = c->colind.as<int32_t>();
+    if (c->opt.spmv_choice >= 1 && c->opt.spmv_choice <= 9) {   // fixed by the caller, no timing
+        const int layout = (c->opt.spmv_choice - 1) / 3, li = (c->opt.spmv_choice - 1) % 3;
+        c->spmv_lanes = lanes[layout][li];
+        c->spmv_layout = layout;
+        c->bicg_lanes = layout == kSpmvFlat ? 16 : lanes[layout][li];
+        c->bicg_chunked = layout == kSpmvChunk;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return CMG_SUCCESS;
+    }
     for (int layout = kSpmvBlock; layout <= kSpmvFlat; layout++)
         for (int li = 0; li < 3; li++) {
             const int l = lanes[layout][li];

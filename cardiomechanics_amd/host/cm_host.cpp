@@ -662,6 +662,7 @@ void CBSolverGPU::Init(const ParameterMap& p, TetgenModel& m) {
     }
     o.precond_reuse = p.Get<bool>("Solver.GPU.PreconditionerReuse", true) ? 1 : 0;
     o.newton_stall = p.Get<int>("Solver.GPU.NewtonStall", o.newton_stall);
+    o.spmv_choice = p.Get<int>("Solver.GPU.SpMVLayout", 0);
     o.transpose_tangent = p.Get<bool>("Solver.GPU.TransposeTangent", false) ? 1 : 0;
     o.pressure_tangent = p.Get<bool>("Solver.GPU.PressureTangent", true) ? 1 : 0;
     o.device = p.Get<int>("Solver.GPU.Device", 0);

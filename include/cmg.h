@@ -162,7 +162,11 @@ typedef struct cmg_options {
     int32_t newton_stall;       /* k > 0: a Newton solve whose residual norm falls by less than 10 %
                                    in each of k consecutive iterations ends as diverged (reason -7)
                                    instead of running on to Solver.MaxIterations; 0: off */
-    int32_t reserved_i2;
+    int32_t spmv_choice;        /* 0 (default): the fastest of the nine SpMV layouts, timed on the
+                                   context's first solve; 1-9: that layout without timing (1-3 one
+                                   3x3 block per step at 4/8/16 lanes, 4-6 144-byte chunks at 4/8/16
+                                   lanes, 7-9 the flat word stream at 16/32/64 lanes), so that two
+                                   contexts do the same arithmetic */
 } cmg_options;
 
 typedef struct cmg_solve_stats {

@@ -73,7 +73,10 @@ def test_cmg_driver_lv_windkessel_matches_python_host(tmp_path, monkeypatch):
     M.write_lat(str(tmp_path / "lv.lat"), mesh.activation)
     xml = os.path.join(ROOT, "cardiomechanics_amd", "configs", "lv_windkessel.xml")
     shutil.copy(xml, tmp_path / "lv.xml")
-    out = subprocess.run([DRIVER, "lv.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    # both hosts use the same SpMV layout (no timing-based choice), so their device
+    # arithmetic is the same call for call
+    out = subprocess.run([DRIVER, "lv.xml", "-parameter", "Solver.GPU.SpMVLayout=3"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=300)
     print(out.stdout[-3000:], out.stderr[-2000:])
     assert out.returncode == 0
     rows = [ln for ln in out.stdout.splitlines() if ln.startswith("t=")]
@@ -88,21 +91,24 @@ def test_cmg_driver_lv_windkessel_matches_python_host(tmp_path, monkeypatch):
     prec = float(p["Solver.Precision"])
     monkeypatch.chdir(tmp_path)   # the circulation writes its result files into the working directory
     with MechanicsContext(m2, [MT.from_parameters(p, 0)], newton_atol=prec, newton_rtol=prec,
-                          newton_stol=prec) as ctx:
+                          newton_stol=prec, spmv_choice=3) as ctx:
         plug = CI.Circulation(p, groups={int(s): k for k, s in enumerate(ctx.group_ids)})
         pressures = []
         recs = SV.run(ctx, "newmark", SV.Timing(float(p["Solver.StartTime"]), float(p["Solver.StopTime"]),
                                                 float(p["Solver.TimeStep"])),
                       plugins=[plug], on_step=lambda rec, st: pressures.append(plug.uP * plug.est[0]))
-    # The two runs make the same calls, but each context tunes its SpMV layout by timing,
-    # so their Krylov iterates can differ in the last bits and a coupling iteration that
-    # lands next to the tolerance may take one repeat more in one of them: both runs
-    # meet |V_FE - V_circ| <= 1e-7 ml at every step, which bounds how far apart they are.
-    print("repeats C++:", c_rep, "Python:", [r.repeats for r in recs])
+    # The two runs make the same device calls with the same SpMV layout; the coupling
+    # arithmetic on the host (C++ and numpy) may differ in the last bits, so a coupling
+    # iteration that lands next to its tolerance could take one repeat more in one run
+    py_rep = [r.repeats for r in recs]
+    py_vol = np.array([r.volumes[0] for r in recs])
+    print("repeats C++:", c_rep, "Python:", py_rep)
+    print(f"pressure rel diff {np.abs(c_p - pressures).max() / np.abs(pressures).max():.2e}, "
+          f"volume rel diff {np.abs(c_vol - py_vol).max() / np.abs(py_vol).max():.2e}")
     assert len(recs) == len(rows)
-    assert abs(sum(c_rep) - sum(r.repeats for r in recs)) <= len(recs) // 2
-    np.testing.assert_allclose(c_p, pressures, rtol=1e-5, atol=1e-3)
-    np.testing.assert_allclose(c_vol, [r.volumes[0] for r in recs], rtol=1e-6)
+    assert sum(a != b for a, b in zip(c_rep, py_rep)) <= 1
+    np.testing.assert_allclose(c_p, pressures, rtol=1e-7, atol=1e-6)
+    np.testing.assert_allclose(c_vol, py_vol, rtol=1e-8)
     assert max(c_p) > 200.0     # the cavity pressure rose past the arterial pressure (2 mmHg)
 
 
