@@ -158,7 +158,8 @@ __global__ __launch_bounds__(256) void k_gather_matrix(int64_t nnzb, const int64
     for (int64_t c = cptr[slot]; c < c1; c++) add_block(s, contrib[c], Ke, Ks, n_solid, npe, flip);
     const int64_t r = blkrow[slot], col = colind[slot];
     if (mval) {
-        const double m = cm * mval[slot];
+#pragma clang fp contract(off)
+        const double m = cm * mval[slot];   // rounded, then added (as the row gather)
 #pragma unroll
         for (int i = 0; i < 3; i++) s[4 * i] += m;
     }
@@ -173,6 +174,143 @@ __global__ __launch_bounds__(256) void k_gather_matrix(int64_t nnzb, const int64
     VT* out = vals + 9 * slot;
 #pragma unroll
     for (int k = 0; k < 9; k++) out[k] = (VT)s[k];
+}
+
+// Row gather: one wave per block row r, four rows per workgroup.  The row's blocks
+// accumulate in LDS (row_cap blocks per wave).  For each solid incidence (e, a) of the
+// row, in element order, K_e(a, 0..npe-1) is one contiguous run of 9 npe values that the
+// first 9 npe / 2 lanes read two values at a time (coalesced, every fetched segment used
+// by this wave) and add at the row positions rpos of the blocks; the surface incidences
+// follow in surface order.  Within one incidence every (block, entry) has one lane, and
+// a wave's LDS operations are carried out in issue order, so each block sums its
+// contributions in the order of its contributor list (LDS adds round like v_add_f64):
+// the values equal the slot gather's bit for bit.  The adds return nothing, so a wave
+// issues them back to back.  The row's column indices, mass scalars and column
+// constraint masks (cmask: the bcmask of each slot's column, for the context's bcmask)
+// are loaded before the element values, so their latency overlaps.  Untransposed
+// maps only (a transposed block of the run would be a 72-byte piece 9 npe values apart).
+constexpr int kRowWaves = 4;
+constexpr int kRowPrefetch = 8;   // incidences loaded before their values are added
+
+template <typename KT, typename VT>
+__global__ __launch_bounds__(64 * kRowWaves) void k_gather_rows(GatherMaps g, const KT* Ke, const double* Ks,
+                                                               const double* mval, double cm,
+                                                               const uint8_t* bcmask, VT* vals, SetStrides st) {
+    extern __shared__ double row_acc[];   // [kRowWaves][row_cap * 9]
+    const int64_t set = blockIdx.y;
+    if (st.active && !st.active[set]) return;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int l = (int)(threadIdx.x & 63);
+    const int64_t r = (int64_t)blockIdx.x * kRowWaves + w;
+    if (r >= g.N) return;   // whole waves: nothing below waits on another wave
+    Ke += set * st.a;
+    if (Ks) Ks += set * st.b;
+    vals += set * st.c;
+    const int npe = g.npe;
+    const int64_t rb = g.rowptr[r];
+    const int nb = (int)(g.rowptr[r + 1] - rb);
+    const int64_t k0 = g.inc_ptr[r], k1 = g.inc_ptr[r + 1];
+    // the row's blocks l and l + 64 (lane per block in the final pass): columns, mass
+    // scalars and column masks, requested before the element values
+    int32_t colA = 0, colB = 0;
+    double mA = 0.0, mB = 0.0;
+    uint8_t cA = 0, cB = 0;
+    if (l < nb) {
+        colA = g.colind[rb + l];
+        if (mval) mA = mval[rb + l];
+        if (bcmask) cA = g.cmask[rb + l];
+    }
+    if (l + 64 < nb) {
+        colB = g.colind[rb + l + 64];
+        if (mval) mB = mval[rb + l + 64];
+        if (bcmask) cB = g.cmask[rb + l + 64];
+    }
+    double* acc = row_acc + (int64_t)w * g.row_cap * 9;
+    for (int t = l; t < 9 * nb; t += 64) acc[t] = 0.0;
+    // solid incidences: lane l < 9 npe / 2 holds entries 2l and 2l + 1 of the run
+    const int nv = 9 * npe;
+    const bool act = 2 * l < nv;
+    const int q0 = act ? 2 * l : 0, q1 = q0 + 1;
+    const int b0 = q0 / 9, e0 = q0 - 9 * b0, b1 = q1 / 9, e1 = q1 - 9 * b1;
+    for (int64_t kb = k0; kb < k1; kb += kRowPrefetch) {
+        // unconditional loads (indices past the row's end repeat its last incidence), so
+        // all of them are in flight together; only the adds below are conditional
+        int64_t code[kRowPrefetch];
+#pragma unroll
+        for (int u = 0; u < kRowPrefetch; u++) code[u] = g.inc[kb + u < k1 ? kb + u : k1 - 1];
+        double v0[kRowPrefetch], v1[kRowPrefetch];
+        int p0[kRowPrefetch], p1[kRowPrefetch];
+#pragma unroll
+        for (int u = 0; u < kRowPrefetch; u++) {
+            const int64_t k = kb + u < k1 ? kb + u : k1 - 1;
+            const KT* run = Ke + code[u] * nv + q0;
+            if constexpr (sizeof(KT) == 8) {
+                const double2 v = *reinterpret_cast<const double2*>(run);   // runs are 16-byte aligned
+                v0[u] = v.x;
+                v1[u] = v.y;
+            } else {
+                const float2 v = *reinterpret_cast<const float2*>(run);     // 8-byte aligned
+                v0[u] = (double)v.x;
+                v1[u] = (double)v.y;
+            }
+            p0[u] = g.rpos[16 * k + b0];
+            p1[u] = g.rpos[16 * k + b1];
+        }
+#pragma unroll
+        for (int u = 0; u < kRowPrefetch; u++) {
+            if (kb + u < k1 && act) {
+                atomicAdd(&acc[9 * p0[u] + e0], v0[u]);
+                atomicAdd(&acc[9 * p1[u] + e1], v1[u]);
+            }
+        }
+    }
+    // surface incidences: the rows 3a..3a+2 of the surface block matrix, 9 nps values
+    // ([i][b][j]), one per lane
+    if (g.sinc_ptr && Ks) {
+        const int nps = g.nps, ns = 9 * nps;
+        const bool sact = l < ns;
+        const int i = sact ? l / (3 * nps) : 0, cc = l - i * 3 * nps;
+        const int sb = cc / 3, sj = cc - 3 * sb;
+        for (int64_t k = g.sinc_ptr[r]; k < g.sinc_ptr[r + 1]; k++) {
+            if (sact) atomicAdd(&acc[9 * g.spos[8 * k + sb] + 3 * i + sj], Ks[(int64_t)g.sinc[k] * ns + l]);
+        }
+    }
+    // + cm M on the diagonal, identity rows / columns of constrained DOFs; lane per block
+    const uint8_t mr = bcmask ? bcmask[r] : 0;
+    for (int kb = l; kb < nb; kb += 64) {
+        const int64_t slot = rb + kb;
+        int32_t col;
+        double m;
+        uint8_t mc;
+        if (kb < 128) {
+            col = kb < 64 ? colA : colB;
+            m = kb < 64 ? mA : mB;
+            mc = kb < 64 ? cA : cB;
+        } else {
+            col = g.colind[slot];
+            m = mval ? mval[slot] : 0.0;
+            mc = bcmask ? g.cmask[slot] : 0;
+        }
+        double v[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = acc[9 * kb + k];
+        if (mval) {
+#pragma clang fp contract(off)
+            const double cmm = cm * m;   // rounded, then added (as the slot gather)
+#pragma unroll
+            for (int i = 0; i < 3; i++) v[4 * i] += cmm;
+        }
+        if (mr | mc) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (((mr >> i) & 1) || ((mc >> j) & 1)) v[3 * i + j] = (r == col && i == j) ? 1.0 : 0.0;
+        }
+        VT* out = vals + 9 * slot;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[k] = (VT)v[k];
+    }
 }
 
 // C = beta_R K + alpha M (Rayleigh damping) in place over K's gathered BSR values, with
@@ -241,6 +379,26 @@ void launch_gather_matrix(const GatherMaps& g, const void* Ke, bool fp32, const 
                           double cm, const uint8_t* bcmask, void* vals, bool vals_fp32, bool flip, hipStream_t s,
                           const SetStrides& b) {
     if (g.nnzb == 0) return;
+    if (g.row_cap > 0 && !g.transposed && !flip && g.rpos && (g.npe == 10 || g.npe == 4)) {
+        const dim3 grid((unsigned)((g.N + kRowWaves - 1) / kRowWaves), b.n_sets);
+        const size_t lds = sizeof(double) * kRowWaves * 9 * (size_t)g.row_cap;
+#define CMG_ROWS(KT, VT)                                                                                   \
+    hipLaunchKernelGGL((k_gather_rows<KT, VT>), grid, dim3(64 * kRowWaves), lds, s, g, (const KT*)Ke, Ks, mval, \
+                       cm, bcmask, (VT*)vals, b)
+        if (fp32) {
+            if (vals_fp32)
+                CMG_ROWS(float, float);
+            else
+                CMG_ROWS(float, double);
+        } else {
+            if (vals_fp32)
+                CMG_ROWS(double, float);
+            else
+                CMG_ROWS(double, double);
+        }
+#undef CMG_ROWS
+        return;
+    }
     const int64_t light_blocks = (g.nnzb + 255) / 256;
     const dim3 grid((unsigned)(light_blocks + (g.n_heavy + 255) / 256), b.n_sets);
 #define CMG_GATHER(KT, VT)                                                                                       \

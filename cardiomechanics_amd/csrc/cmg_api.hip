@@ -425,6 +425,34 @@ int cmg_create(const cmg_mesh* mesh, const cmg_material* materials, int32_t n_ma
                     const int64_t sl = sslot[(si * nps + a) * nps + b];
                     contrib[fillp[sl]++] = (int32_t)(ns_blocks + ((((si * nps + a) * nps + b) << 3) | nps));
                 }
+        // row gather tables: position within its row of the block each incidence's local
+        // nodes map to (element order and surface order, as the contributor lists)
+        int64_t longest = 0;
+        for (int64_t r = 0; r < N; r++) longest = std::max(longest, c->h_rowptr[r + 1] - c->h_rowptr[r]);
+        const int64_t cap = (longest + 7) / 8 * 8;
+        // (one surface value per lane: 9 nps <= 64)
+        c->row_cap = (cap <= kRowGatherMaxCap && npe <= 16 && 9 * nps <= 64) ? (int)cap : 0;
+        if (c->row_cap > 0) {
+            std::vector<uint8_t> rpos(inc.size() * 16, 0);
+            for (size_t k = 0; k < inc.size(); k++) {
+                const int64_t r = mesh->elements[inc[k]];
+                for (int b = 0; b < npe; b++)
+                    rpos[16 * k + b] = (uint8_t)(slot_of[(int64_t)inc[k] * npe + b] - c->h_rowptr[r]);
+            }
+            CMG_TRY(upload(c->rpos, rpos.data(), rpos.size(), s));
+            if (c->S > 0) {
+                std::vector<uint8_t> spos(sinc.size() * 8, 0);
+                for (size_t k = 0; k < sinc.size(); k++) {
+                    const int64_t r = mesh->surfaces[sinc[k]];
+                    for (int b = 0; b < nps; b++)
+                        spos[8 * k + b] = (uint8_t)(sslot[(int64_t)sinc[k] * nps + b] - c->h_rowptr[r]);
+                }
+                CMG_TRY(upload(c->spos, spos.data(), spos.size(), s));
+            }
+            std::vector<uint8_t> cmask(nnzb);
+            for (int64_t k = 0; k < nnzb; k++) cmask[k] = c->h_bc[c->h_colind[k]];
+            CMG_TRY(upload(c->cmask, cmask.data(), cmask.size(), s));
+        }
         std::vector<int32_t> heavy;
         for (int64_t k = 0; k < nnzb; k++)
             if (cnt[k + 1] - cnt[k] > kHeavyContrib) heavy.push_back((int32_t)k);
@@ -573,6 +601,18 @@ GatherMaps gather_maps(cmg_ctx* c) {
     g.npe = c->npe;
     g.blkrow = c->blkrow.as<int32_t>();
     g.colind = c->colind.as<int32_t>();
+    g.transposed = c->maps_transposed;
+    g.N = c->N;
+    g.nps = c->nps;
+    g.row_cap = c->row_cap;
+    g.rowptr = c->rowptr.as<int64_t>();
+    g.inc_ptr = c->inc_ptr.as<int64_t>();
+    g.inc = c->inc.as<int32_t>();
+    g.sinc_ptr = c->S > 0 ? c->sinc_ptr.as<int64_t>() : nullptr;
+    g.sinc = c->S > 0 ? c->sinc.as<int32_t>() : nullptr;
+    g.rpos = c->rpos.as<uint8_t>();
+    g.spos = c->spos.as<uint8_t>();
+    g.cmask = c->cmask.as<uint8_t>();
     return g;
 }
 

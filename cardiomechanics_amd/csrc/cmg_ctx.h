@@ -95,6 +95,8 @@ struct cmg_ctx {
     int64_t n_heavy = 0;
     bool maps_transposed = false;   // T10 blocks enter the assembled tangent as K_e^T
     cmg::DevBuf inc_ptr, inc, sinc_ptr, sinc;
+    cmg::DevBuf rpos, spos, cmask;  // row positions and column masks for the row gather (GatherMaps)
+    int row_cap = 0;
     // state and work vectors
     cmg::DevBuf Xref, nodes, xeval, d, dpred, vpred, vel, acc, r, delta, dsave, tmpA, tmpB;
     cmg::DevBuf vals, dinv;

@@ -132,7 +132,25 @@ struct GatherMaps {
     int npe = 10;
     const int32_t* blkrow = nullptr;   // [nnzb] block row of each slot
     const int32_t* colind = nullptr;   // [nnzb]
+    // Row gather (untransposed maps): the incidence lists of the residual gather give
+    // each block row's (element, local node) pairs in element order and its (surface,
+    // local node) pairs in surface order -- the order of every slot's contributor list
+    // -- and rpos / spos the position within the row of the block of each local node.
+    // row_cap = 0 when a row is longer than the gather's LDS holds.
+    bool transposed = false;           // maps hold the transposed T10 convention
+    int64_t N = 0;
+    int nps = 3;
+    int row_cap = 0;                   // longest block row, rounded up to 8
+    const int64_t* rowptr = nullptr;   // [N + 1]
+    const int64_t* inc_ptr = nullptr;  // [N + 1] solid incidences (e * npe + a)
+    const int32_t* inc = nullptr;
+    const int64_t* sinc_ptr = nullptr; // [N + 1] surface incidences (s * nps + a), or nullptr
+    const int32_t* sinc = nullptr;
+    const uint8_t* rpos = nullptr;     // [n_inc][16] row position of the block of local node b
+    const uint8_t* spos = nullptr;     // [n_sinc][8]
+    const uint8_t* cmask = nullptr;    // [nnzb] constraint mask of each slot's column node
 };
+constexpr int kRowGatherMaxCap = 224;   // 4 waves x 224 blocks x 72 B of LDS
 // BSR values from element/surface contributions + cm*M + identity on constrained DOFs.
 // flip = 1 assembles every solid element block as K_e^T instead (the reference's T10
 // finite-difference layout when the maps hold the untransposed convention, and back).

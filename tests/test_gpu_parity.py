@@ -171,6 +171,47 @@ def test_transposed_assembly_convention():
     np.testing.assert_array_equal(Kb, Ka.T)
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_matrix_gather_is_the_in_order_sum_of_the_element_tangents(precision):
+    """Each assembled block is its element blocks summed in FP64 in element order, then
+    its surface blocks (zeros without the pressure tangent), rounded once to the value
+    type; constrained rows and columns hold the identity.  Formed on the host from the
+    device's own element tangents, that sum equals the device's values bit for bit."""
+    mesh, mats = lv_small()
+    x = deformed(mesh).ravel()
+    with MechanicsContext(mesh, mats, precision=precision) as ctx:
+        vals = ctx.tangent(x, 0.12, include_pressure=False).reshape(-1, 3, 3)
+        Ke = ctx.element_tangents(x, 0.12)
+        rowptr, colind = ctx.rowptr.copy(), ctx.colind.copy()
+    N = mesh.n_nodes
+    E, npe = mesh.elements.shape
+    keys = np.repeat(np.arange(N, dtype=np.int64), np.diff(rowptr)) * N + colind
+    assert np.all(np.diff(keys) > 0)
+
+    def slots(conn):
+        k = conn.shape[1]
+        r = np.repeat(conn.astype(np.int64), k, axis=1).ravel()
+        c = np.tile(conn.astype(np.int64), (1, k)).ravel()
+        s = np.searchsorted(keys, r * N + c)
+        assert np.array_equal(keys[s], r * N + c)
+        return s
+
+    blocks = Ke.astype(np.float64).reshape(E, npe, 3, npe, 3).transpose(0, 1, 3, 2, 4).reshape(-1, 3, 3)
+    acc = np.zeros((colind.size, 3, 3))
+    np.add.at(acc, slots(mesh.elements), blocks)          # sequential: element order per block
+    np.add.at(acc, slots(mesh.surfaces), 0.0)
+    blkrow = np.repeat(np.arange(N), np.diff(rowptr))
+    bc = mesh.bc.astype(np.int64)
+    for i in range(3):
+        for j in range(3):
+            m = ((bc[blkrow] >> i) & 1).astype(bool) | ((bc[colind] >> j) & 1).astype(bool)
+            acc[m, i, j] = np.where(blkrow[m] == colind[m], 1.0 if i == j else 0.0, 0.0)
+    want = acc.astype(vals.dtype)
+    assert (bc != 0).any()
+    print(f"precision {precision}: {colind.size} blocks, {np.count_nonzero(want != vals)} differ")
+    assert want.tobytes() == vals.tobytes()
+
+
 @pytest.mark.parametrize("surface_type", [3, 6])
 def test_surface_forces_and_cavity_volume(surface_type):
     mesh, mats = lv_small(surface_type=surface_type)
