@@ -160,11 +160,11 @@ def lv200k():
 
 @pytest.fixture(scope="module")
 def lv200k_windows(lv200k):
-    """the first 8 ms of the 200,340-element LV beat (activation onset at ~5 ms), FP64 and
-    mixed precision, Newton tolerance 1e-12"""
+    """the 0-10 ms window of the 200,340-element LV beat (activation onset at ~5 ms) that
+    the benchmark measures, FP64 and mixed precision, Newton tolerance 1e-12"""
     mesh, _ = lv200k
     tol = (1e-12, 1e-12, 1e-14)
-    return tol, {prec: newmark_window(mesh, prec, tol, 0.008) for prec in (0, 1)}
+    return tol, {prec: newmark_window(mesh, prec, tol, 0.010) for prec in (0, 1)}
 
 
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
@@ -179,7 +179,8 @@ def test_lv200k_beat_window_steps_are_oracle_roots(lv200k, lv200k_windows, prec)
 
 def test_lv200k_mixed_precision_displacements_match_fp64(lv200k, lv200k_windows):
     """The 200k-element LV beat: at every time both runs accepted, the mixed-precision nodes differ from
-    the FP64 nodes by <= 1e-8 of the largest displacement in the window."""
+    the FP64 nodes by <= 1e-8 of the largest displacement in the window.  (Both are roots to
+    1e-12 N; two FP64 solves with different preconditioners differ by ~1.2e-8 here.)"""
     mesh, _ = lv200k
     tol, runs = lv200k_windows
     X0 = mesh.coords.ravel()
