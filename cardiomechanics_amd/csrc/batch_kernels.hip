@@ -65,11 +65,28 @@ __global__ __launch_bounds__(kThreads) void k_bcg_spmv_dot(int64_t N, const int6
     if (threadIdx.x == 0) partial[set * nb + blockIdx.x] = tot;
 }
 
-// sum of nb partials (stride `ld` between sets, `k` consecutive values per block)
+// sum of nb partials (stride `ld` between sets, `k` consecutive values per block);
+// eight independent loads in flight per thread, summed in a fixed order
 __device__ __forceinline__ double reduce_partials(const double* part, int nb, int k, int j, double* sh) {
     double v = 0.0;
+#pragma unroll 8
     for (int b = threadIdx.x; b < nb; b += kThreads) v += part[(int64_t)b * k + j];
     return block_sum256(v, sh);
+}
+
+// the K sums of nb partials of K consecutive values each, in one pass over contiguous
+// memory (thread t takes blocks t, t + 256, ... in order, then the fixed block tree)
+template <int K>
+__device__ __forceinline__ void reduce_partials_k(const double* part, int nb, double* sh, double* out) {
+    double v[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) v[j] = 0.0;
+#pragma unroll 4
+    for (int b = threadIdx.x; b < nb; b += kThreads)
+#pragma unroll
+        for (int j = 0; j < K; j++) v[j] += part[(int64_t)b * K + j];
+#pragma unroll
+    for (int j = 0; j < K; j++) out[j] = block_sum256(v[j], sh);
 }
 
 __global__ __launch_bounds__(kThreads) void k_bcg_alpha(BatchCG cg, int nb) {
@@ -174,8 +191,9 @@ __global__ __launch_bounds__(kThreads) void k_bcg_beta(BatchCG cg, int nb) {
     const int set = blockIdx.x;
     if (cg.done[set]) return;
     const double* part = cg.part + (int64_t)set * cg.part_ld;
-    const double rz = reduce_partials(part, nb, 2, 0, sh);
-    const double rr = reduce_partials(part, nb, 2, 1, sh);
+    double t2[2];
+    reduce_partials_k<2>(part, nb, sh, t2);
+    const double rz = t2[0], rr = t2[1];
     if (threadIdx.x == 0) {
         cg.it[set] += 1;
         cg.rr[set] = rr;
@@ -251,8 +269,9 @@ __global__ __launch_bounds__(kThreads) void k_bcg_init_reduce(BatchCG cg, int nb
         return;
     }
     const double* part = cg.part + (int64_t)set * cg.part_ld;
-    const double rz = reduce_partials(part, nb, 2, 0, sh);
-    const double rr = reduce_partials(part, nb, 2, 1, sh);
+    double t2[2];
+    reduce_partials_k<2>(part, nb, sh, t2);
+    const double rz = t2[0], rr = t2[1];
     if (threadIdx.x == 0) {
         cg.rz[set] = rz;
         cg.rr[set] = rr;
@@ -481,7 +500,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_omega(BatchBiCG k, int nb) {
     if (k.done[set]) return;
     const double* part = k.part + (int64_t)set * k.part_ld;
     double d[kBicgDots];
-    for (int j = 0; j < kBicgDots; j++) d[j] = reduce_partials(part, nb, kBicgDots, j, sh);
+    reduce_partials_k<kBicgDots>(part, nb, sh, d);
     if (threadIdx.x == 0) {
         const double ts = d[0], tt = d[1], rht = d[2], rhs = d[3], ss = d[4];
         const double om = (tt > 0.0) ? ts / tt : 0.0;
@@ -555,8 +574,9 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init_reduce(BatchBiCG k, int 
         return;
     }
     const double* part = k.part + (int64_t)set * k.part_ld;
-    const double rr = reduce_partials(part, nb, 2, 0, sh);
-    const double bb = reduce_partials(part, nb, 2, 1, sh);
+    double t2[2];
+    reduce_partials_k<2>(part, nb, sh, t2);
+    const double rr = t2[0], bb = t2[1];
     if (threadIdx.x == 0) {
         const double bt = (bb > 0.0) ? bb : 1.0;
         k.rho[set] = rr;
