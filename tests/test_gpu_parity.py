@@ -171,13 +171,17 @@ def test_transposed_assembly_convention():
     np.testing.assert_array_equal(Kb, Ka.T)
 
 
+@pytest.mark.parametrize("case", ["lv_t10", "cube_t4"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
-def test_matrix_gather_is_the_in_order_sum_of_the_element_tangents(precision):
+def test_matrix_gather_is_the_in_order_sum_of_the_element_tangents(precision, case):
     """Each assembled block is its element blocks summed in FP64 in element order, then
     its surface blocks (zeros without the pressure tangent), rounded once to the value
     type; constrained rows and columns hold the identity.  Formed on the host from the
     device's own element tangents, that sum equals the device's values bit for bit."""
-    mesh, mats = lv_small()
+    if case == "lv_t10":
+        mesh, mats = lv_small()
+    else:
+        mesh, mats = M.cube_mesh(3, order=4), [MT.cube_guccione()]
     x = deformed(mesh).ravel()
     with MechanicsContext(mesh, mats, precision=precision) as ctx:
         vals = ctx.tangent(x, 0.12, include_pressure=False).reshape(-1, 3, 3)
