@@ -17,7 +17,8 @@ Parity status of each oracle function (what pins it; see tests/test_oracle_*.py)
                             to FD noise
   T10/T4 complex-step tangent  exact to rounding by construction; symmetric for
                             conservative laws
-  mass matrices             sum = rho V per direction; lumped rho V / 10
+  mass matrices             sum = rho V per direction; lumped rho V / 10 (T10), the row
+                            sums of the consistent mass (T4)
   T3/T6 forces, volume      cube face force = p A n; closed-surface volume independent
                             of the reference point; tetra volume closed form
   Newton / Newmark drivers  linear spring-mass closed form; residual at convergence
@@ -122,6 +123,7 @@ def lib():
             "orc_t4_cs_tangent": (ctypes.c_int32, [_D, _D, _D, ctypes.c_double, _U8, _M, ctypes.c_double,
                                                    ctypes.c_double, _D]),
             "orc_t4_mass_consistent": (None, [ctypes.c_double, ctypes.c_double, _U8, _D]),
+            "orc_t4_mass_lumped": (None, [ctypes.c_double, ctypes.c_double, _U8, _D]),
             "orc_t3_pressure_forces": (None, [_D, _U8, ctypes.c_double, _D]),
             "orc_t3_pressure_cs_tangent": (None, [_D, _U8, ctypes.c_double, _D]),
             "orc_t3_volume": (ctypes.c_double, [_D, _D]),
@@ -338,6 +340,13 @@ def t4_mass_consistent(volume, density, bc):
     return M.reshape(12, 12)
 
 
+def t4_mass_lumped(volume, density, bc):
+    bc, pbc = _u8(bc)
+    M = np.zeros(144)
+    lib().orc_t4_mass_lumped(volume, density, pbc, M.ctypes.data_as(_D))
+    return M.reshape(12, 12)
+
+
 def surface_forces(x, bc, p):
     x, px = _d(x)
     bc, pbc = _u8(bc)
@@ -401,7 +410,7 @@ class ElementData:
 
 def mass_batch(ed: ElementData, density, consistent=True):
     """element mass matrices [E][3npe][3npe] (t10_mass_consistent / t10_mass_lumped /
-    t4_mass_consistent of every element); density per element"""
+    t4_mass_consistent / t4_mass_lumped of every element); density per element"""
     E = ed.conn.shape[0]
     nd = 3 * ed.npe
     Me = np.zeros((E, nd, nd))

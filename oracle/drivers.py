@@ -62,13 +62,11 @@ class CpuMechanics:
     # -- element / surface loops ------------------------------------------------
     def _mass(self, consistent, chunk=50_000):
         """Assembled mass matrix from the oracle's element mass matrices (t10_mass_consistent
-        / t10_mass_lumped / t4_mass_consistent).  Each is m_ab (x) I3 with the rows of
+        / t10_mass_lumped / t4_mass_consistent / t4_mass_lumped).  Each is m_ab (x) I3 with the rows of
         constrained DOFs zeroed, so the nodal scalars m_ab of the unconstrained element
         matrices are assembled (in chunks of elements) and the constrained rows cleared
         once: M = D_free (m (x) I3)."""
         rho = np.array([self.materials[k].density for k in self.mesh.material], dtype=np.float64)
-        if self.mesh.npe == 4:
-            consistent = True          # the T4 mass restated here is the consistent one
         free_ed = O.ElementData.__new__(O.ElementData)
         free_ed.__dict__.update(self.ed.__dict__)
         free_ed.bcmask = np.zeros_like(self.ed.bcmask)

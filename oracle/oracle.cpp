@@ -920,6 +920,14 @@ void orc_t4_mass_consistent(double volume, double density, const uint8_t* bc12, 
                         ((k == l) && (bc12[3 * i + k] == 0)) ? ((i == j) ? 2.0 : 1.0) * c : 0.0;
 }
 
+// CBElementSolidT4 lumped mass (CBElementSolidT4.cpp:432-456): rho V / 4 on every
+// diagonal entry (the row sums of the consistent T4 mass), rows of constrained DOFs zero
+void orc_t4_mass_lumped(double volume, double density, const uint8_t* bc12, double* M144) {
+    double c = (density * volume) / 4;
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 12; j++) M144[12 * i + j] = ((i == j) && (bc12[i] == 0)) ? c : 0.0;
+}
+
 void orc_t3_pressure_forces(const double* x9, const uint8_t* bc9, double p, double* f9) {
     t3_forces<double>(x9, bc9, p, f9);
 }
@@ -1157,7 +1165,10 @@ void orc_mass_batch(int32_t npe, int64_t n_elem, const int32_t* conn, const doub
             else
                 orc_t10_mass_lumped(w[e] / 6, density[e], bc, out);
         } else {
-            orc_t4_mass_consistent(w[e], density[e], bc, out);
+            if (consistent)
+                orc_t4_mass_consistent(w[e], density[e], bc, out);
+            else
+                orc_t4_mass_lumped(w[e], density[e], bc, out);
         }
     }
 }

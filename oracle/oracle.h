@@ -114,6 +114,7 @@ int32_t orc_t4_cs_tangent(const double* x12, const double* dNdX12, const double*
                           double volume, const uint8_t* bc12, const orc_material* m,
                           double time, double tact, double* K144);
 void orc_t4_mass_consistent(double volume, double density, const uint8_t* bc12, double* M144);
+void orc_t4_mass_lumped(double volume, double density, const uint8_t* bc12, double* M144);
 
 /* ---- surface loads (CBElementSurfaceT3.cpp / CBElementSurfaceT6.cpp) -- */
 void orc_t3_pressure_forces(const double* x9, const uint8_t* bc9, double p, double* f9);

@@ -233,10 +233,12 @@ def test_surface_forces_and_cavity_volume(surface_type):
     assert vg[0] < 0   # endocardial normals point into the cavity, as in the reference's ellipsoid.sur
 
 
+@pytest.mark.parametrize("case", ["lv_t10", "cube_t4"])
 @pytest.mark.parametrize("consistent", [1, 0], ids=["consistent", "lumped"])
-def test_mass_matrix(consistent):
-    """consistent (CBElementSolidT10.cpp:352-394) or lumped (:396-422) mass"""
-    mesh, mats = lv_small()
+def test_mass_matrix(consistent, case):
+    """consistent (CBElementSolidT10.cpp:352-394, CBElementSolidT4.cpp:365-410) or lumped
+    (CBElementSolidT10.cpp:396-422, CBElementSolidT4.cpp:432-456) mass"""
+    mesh, mats = lv_small() if case == "lv_t10" else (M.cube_mesh(2, order=4), [MT.cube_guccione()])
     cpu = CpuMechanics(mesh, mats, consistent_mass=bool(consistent))
     with MechanicsContext(mesh, mats, consistent_mass=consistent) as ctx:
         Mg = ctx.bsr(ctx.mass_matrix()).toarray()
@@ -254,6 +256,32 @@ def test_t4_static_load_steps_match_oracle():
     for t, err, nit, kit in res:
         print(f"T4 t={t:.2f} newton={nit} krylov={kit} rel err={err:.2e}")
         assert err <= 1e-8
+
+
+def test_t4_newmark_steps_with_lumped_mass_match_oracle():
+    """Linear tetrahedra in Newmark steps (beta=1/4, gamma=1/2) with the lumped mass
+    (Solver.NewmarkBeta.ConsistentMassMatrix = false, CBElementSolidT4.cpp:432-456) under
+    a pressure ramp on the cube."""
+    mesh, mats = M.cube_mesh(2, order=4), [MT.cube_guccione()]
+    cpu = CpuMechanics(mesh, mats, consistent_mass=False)
+    dt = 0.005
+    with MechanicsContext(mesh, mats, consistent_mass=0, newton_atol=1e-12, newton_rtol=1e-12,
+                          newton_stol=1e-15) as ctx:
+        for k in range(1, 5):
+            t, p = k * dt, 500.0 * k
+            ctx.set_pressures([p])
+            cpu.pressures[:] = p
+            rc, st = ctx.newmark_step(t, dt)
+            assert rc == 2, st.as_dict()
+            cpu.newmark_step(t, dt)
+            g = ctx.get_state()["nodes"]
+            u = cpu.nodes - mesh.coords.ravel()
+            err = np.linalg.norm(g - cpu.nodes) / np.linalg.norm(u)
+            vg, vo = ctx.cavity_volumes()[0], cpu.cavity_volumes()[0]
+            print(f"T4 lumped t={t:.3f} newton={st.newton_iterations} disp err={err:.2e} "
+                  f"vol err={abs(vg - vo) / abs(vo):.2e}")
+            assert err <= 1e-8
+            assert abs(vg - vo) <= 1e-8 * abs(vo)
 
 
 @pytest.mark.parametrize("pc", [1, 2], ids=["block_jacobi", "schwarz"])

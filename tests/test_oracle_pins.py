@@ -393,6 +393,13 @@ def test_mass_matrices_sum_to_element_mass():
     assert np.abs(O.t10_mass_consistent(detJ, rho, bc)[7]).max() == 0
     M4 = O.t4_mass_consistent(V, rho, np.zeros(12, np.uint8))
     assert M4[0::3, 0::3].sum() == pytest.approx(rho * V)
+    # T4 lumped: the row sums of the consistent mass on the diagonal (row-sum lumping)
+    L4 = O.t4_mass_lumped(V, rho, np.zeros(12, np.uint8))
+    np.testing.assert_allclose(L4, np.diag(M4.sum(axis=1)), rtol=1e-15)
+    assert L4[0::3, 0::3].sum() == pytest.approx(rho * V)
+    bc4 = np.zeros(12, np.uint8)
+    bc4[4] = 1
+    assert O.t4_mass_lumped(V, rho, bc4)[4, 4] == 0
 
 
 # ---------------------------------------------------------------- surfaces
