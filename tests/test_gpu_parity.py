@@ -159,6 +159,30 @@ def test_assembled_residual_and_tangent(case):
     assert np.linalg.norm(Kg - Ko) <= 1e-12 * np.linalg.norm(Ko)
 
 
+def test_assembled_tangent_without_the_pressure_tangent():
+    """pressure_tangent = 0: the surface pressure adds its forces but no tangent blocks
+    (the reference's T3 surface sets no matrix entries); the residual keeps the load."""
+    mesh, mats = lv_small()
+    x = deformed(mesh)
+    t, p = 0.12, -4000.0
+    cpu = CpuMechanics(mesh, mats, pressure_tangent=False)
+    cpu.pressures[:] = p
+    with MechanicsContext(mesh, mats, pressure_tangent=0) as ctx:
+        ctx.set_pressures([p])
+        rg = ctx.residual(x.ravel(), t)
+        Kg = ctx.bsr(ctx.tangent(x.ravel(), t)).toarray()
+    ro = cpu.residual(x.ravel(), t)
+    Ko = cpu.tangent(x.ravel(), t).toarray()
+    cpu_p = CpuMechanics(mesh, mats)
+    cpu_p.pressures[:] = p
+    dK = np.linalg.norm(cpu_p.tangent(x.ravel(), t).toarray() - Ko) / np.linalg.norm(Ko)
+    assert dK > 1e-8          # the pressure tangent is large enough for the check to see it
+    print(f"pressure tangent {dK:.2e} of ||K||; without it: residual err "
+          f"{np.abs(rg - ro).max() / np.abs(ro).max():.2e}, tangent err {np.linalg.norm(Kg - Ko) / np.linalg.norm(Ko):.2e}")
+    assert np.abs(rg - ro).max() <= 1e-12 * np.abs(ro).max()
+    assert np.linalg.norm(Kg - Ko) <= 1e-12 * np.linalg.norm(Ko)
+
+
 def test_transposed_assembly_convention():
     """transpose_tangent=1 assembles each solid element block transposed, the T10
     reference convention (CBElementSolidT10.cpp:559-565)."""
