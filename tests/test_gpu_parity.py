@@ -137,9 +137,13 @@ def test_mixed_precision_element_tangents():
     assert (np.abs(fg - fo) / np.abs(fo).max(axis=1, keepdims=True)).max() <= 1e-12   # residual stays FP64
 
 
-@pytest.mark.parametrize("case", ["cube_guccione", "lv_holzapfel_active", "ellipsoid_fixture_holzapfel"])
+ASSEMBLY_CASES = dict(CASES, lv_holzapfel_t6=lambda: lv_small(surface_type=6))
+
+
+@pytest.mark.parametrize("case", ["cube_guccione", "lv_holzapfel_active", "ellipsoid_fixture_holzapfel",
+                                  "lv_holzapfel_t6"])
 def test_assembled_residual_and_tangent(case):
-    mesh, mats = CASES[case]()
+    mesh, mats = ASSEMBLY_CASES[case]()
     x = deformed(mesh)
     t = 0.12
     p = -4000.0
@@ -191,8 +195,15 @@ def test_transposed_assembly_convention():
     with MechanicsContext(mesh, mats) as a, MechanicsContext(mesh, mats, transpose_tangent=1) as b:
         Ka = a.bsr(a.tangent(x.ravel(), 0.12, include_pressure=False)).toarray()
         Kb = b.bsr(b.tangent(x.ravel(), 0.12, include_pressure=False)).toarray()
+        # the surface pressure blocks enter untransposed in either convention
+        for ctx in (a, b):
+            ctx.set_pressures([-4000.0])
+        Pa = a.bsr(a.tangent(x.ravel(), 0.12)).toarray() - Ka
+        Pb = b.bsr(b.tangent(x.ravel(), 0.12)).toarray() - Kb
     assert np.linalg.norm(Ka - Ka.T) > 1e-3 * np.linalg.norm(Ka)       # Holzapfel as coded is unsymmetric
     np.testing.assert_array_equal(Kb, Ka.T)
+    assert np.linalg.norm(Pa) > 1e-6 * np.linalg.norm(Ka)
+    assert np.linalg.norm(Pb - Pa) <= 1e-12 * np.linalg.norm(Ka)
 
 
 @pytest.mark.parametrize("case", ["lv_t10", "cube_t4"])
