@@ -512,6 +512,38 @@ def test_newmark_steps_with_rayleigh_damping_match_oracle(precision):
             assert abs(ed - cpu.damping_energy) <= 100 * tol * abs(cpu.damping_energy)
 
 
+def test_initial_guess_overrides_the_predictor_once():
+    """cmg_set_initial_guess: the next step's Newton starts from the given increment
+    (guess_source 1) -- here the converged increment of the same step, so Newton needs
+    at most one iteration and lands on the same root -- and the step after that uses the
+    predictor again (guess_source 0)."""
+    mesh, mats = lv_small()
+    tol = dict(newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15)
+    dt = 0.005
+    plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
+    p1, p2 = plug.apply(dt, dt)[0], plug.apply(2 * dt, dt)[0]
+    with MechanicsContext(mesh, mats, **tol) as a:
+        a.set_pressures([p1])
+        rc, sa = a.newmark_step(dt, dt)
+        assert rc == 2
+        st_a = a.get_state()
+    with MechanicsContext(mesh, mats, **tol) as b:
+        b.set_pressures([p1])
+        b.set_initial_guess(st_a["increment"])
+        rc, sb = b.newmark_step(dt, dt)
+        assert rc == 2 and sb.guess_source == 1, sb.as_dict()
+        st_b = b.get_state()
+        u = st_a["nodes"] - mesh.coords.ravel()
+        err = np.linalg.norm(st_b["nodes"] - st_a["nodes"]) / np.linalg.norm(u)
+        print(f"Newton iterations: predictor {sa.newton_iterations}, given guess {sb.newton_iterations}; "
+              f"rel diff {err:.2e}")
+        assert sb.newton_iterations <= 1 and sb.newton_iterations < sa.newton_iterations
+        assert err <= 1e-10
+        b.set_pressures([p2])
+        rc, sc = b.newmark_step(2 * dt, dt)
+        assert rc == 2 and sc.guess_source == 0, sc.as_dict()
+
+
 def test_newton_reports_a_krylov_solve_without_update_as_failure():
     """A Krylov solve that returns no update (here: krylov_maxit = 0) ends the step as
     SNES_DIVERGED_LINEAR_SOLVE (-3); it must not read as a short step (step-norm
