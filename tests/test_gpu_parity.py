@@ -512,6 +512,31 @@ def test_newmark_steps_with_rayleigh_damping_match_oracle(precision):
             assert abs(ed - cpu.damping_energy) <= 100 * tol * abs(cpu.damping_energy)
 
 
+def test_spmv_layout_tuning_and_fixed_choice():
+    """The context times the nine SpMV layouts on its first solve and keeps the
+    fastest (cmg_spmv_tuning); spmv_choice fixes one without timing.  Both solve the
+    same static step to the Newton tolerance."""
+    mesh, mats = lv_small()
+    tol = dict(newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15)
+    res = {}
+    for choice in (0, 3):
+        with MechanicsContext(mesh, mats, spmv_choice=choice, **tol) as ctx:
+            ctx.set_pressures([-300.0])
+            rc, st = ctx.static_step(1.0, 1.0)
+            assert rc == 2
+            ms, layout, lanes = ctx.spmv_tuning()
+            res[choice] = ctx.get_state()["nodes"]
+        print(f"spmv_choice {choice}: layout {layout}, lanes {lanes}, ms {np.round(ms, 4)}")
+        if choice == 0:
+            assert np.all(ms > 0)
+            k = int(np.argmin(ms))
+            assert layout == k // 3 and lanes == ((4, 8, 16) if k < 6 else (16, 32, 64))[k % 3]
+        else:
+            assert layout == 0 and lanes == 16 and np.all(ms == 0)
+    u = res[0] - mesh.coords.ravel()
+    assert np.linalg.norm(res[3] - res[0]) <= 1e-8 * np.linalg.norm(u)
+
+
 def test_initial_guess_overrides_the_predictor_once():
     """cmg_set_initial_guess: the next step's Newton starts from the given increment
     (guess_source 1) -- here the converged increment of the same step, so Newton needs
