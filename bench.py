@@ -72,8 +72,10 @@ def main():
                     help="CPU baseline sample: repeat 20k-element chunks of the mesh for about this long")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-element", type=int, default=0,
-                    help="profiling mode: N launches of the element kernel (default variant) and of a "
-                         "known-size copy kernel, nothing else")
+                    help="profiling mode: N launches of the element kernel (default variant, or "
+                         "--profile-variant) and of a known-size copy kernel, nothing else")
+    ap.add_argument("--profile-variant", type=int, default=0,
+                    help="element kernel variant of --profile-element (0: the precision's default)")
     ap.add_argument("--profile-assembly", type=int, default=0,
                     help="profiling mode: N passes of element kernel + deterministic assembly "
                          "(cmg_residual_tangent_device) and N launches of a known-size copy kernel, nothing else")
@@ -186,7 +188,7 @@ def main():
         ctx.close()
         return
     if args.profile_element > 0:
-        ctx = MechanicsContext(mesh, mats, precision=prec)   # the default variant of the precision
+        ctx = MechanicsContext(mesh, mats, precision=prec, element_variant=args.profile_variant)
         xd = DeviceArray(x.ravel())
         ncopy = 64 * 1024 * 1024
         src = DeviceArray(np.ones(ncopy))
