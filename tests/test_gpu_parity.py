@@ -293,6 +293,33 @@ def test_t4_static_load_steps_match_oracle():
         assert err <= 1e-8
 
 
+def test_newmark_steps_with_the_beat_parameters_match_oracle():
+    """Newmark with beta = 0.3025, gamma = 0.6 (the LV beat's settings): predictor and
+    corrector (CBSolverNewmarkBeta.cpp:845-852, 984-1015) depend on gamma only away from
+    the trapezoidal rule, so nodes, velocities and accelerations are all compared."""
+    mesh, mats = lv_small()
+    beta, gamma, dt = 0.3025, 0.6, 0.005
+    cpu = CpuMechanics(mesh, mats)
+    plug = SV.sinus_pulse(-16000.0, 0.0, 0.8)
+    with MechanicsContext(mesh, mats, newton_atol=1e-12, newton_rtol=1e-12, newton_stol=1e-15,
+                          newmark_beta=beta, newmark_gamma=gamma) as ctx:
+        for k in range(1, 7):
+            t = k * dt
+            p = plug.apply(t, dt)[0]
+            ctx.set_pressures([p])
+            cpu.pressures[:] = p
+            rc, st = ctx.newmark_step(t, dt)
+            assert rc == 2, st.as_dict()
+            cpu.newmark_step(t, dt, beta=beta, gamma=gamma)
+            g = ctx.get_state()
+            errs = [np.linalg.norm(g["nodes"] - cpu.nodes) / np.linalg.norm(cpu.nodes - mesh.coords.ravel()),
+                    np.linalg.norm(g["velocity"] - cpu.vel) / np.linalg.norm(cpu.vel),
+                    np.linalg.norm(g["acceleration"] - cpu.acc) / np.linalg.norm(cpu.acc)]
+            print(f"t={t:.3f} newton={st.newton_iterations} rel err disp {errs[0]:.2e} vel {errs[1]:.2e} "
+                  f"acc {errs[2]:.2e}")
+            assert max(errs) <= 1e-8
+
+
 def test_t4_newmark_steps_with_lumped_mass_match_oracle():
     """Linear tetrahedra in Newmark steps (beta=1/4, gamma=1/2) with the lumped mass
     (Solver.NewmarkBeta.ConsistentMassMatrix = false, CBElementSolidT4.cpp:432-456) under
